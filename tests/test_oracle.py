@@ -1,0 +1,225 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each test checks the oracle against something it did not compute itself:
+the reference's own test energies, a closed form, an independent brute-force
+formula, or finite differences.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import lagrange, triples
+from oracle.ccsd import (ECInfo, DFInts, ccdriver, calc_cc_resid, calc_dressed_ints,
+                         calc_doubles_energy, calc_singles_energy, calc_fock_matrix,
+                         calc_HF_energy)
+from oracle.fcidump import read_fcidump
+from oracle.synthetic import make_df_system, splitmix64, write_fcidump
+
+H2O = os.path.join(GOLDEN, "H2O.FCIDUMP.gz")
+H2O_ST1 = os.path.join(GOLDEN, "H2O_ST1.FCIDUMP.gz")
+EPS_REF = 1e-6          # tolerance of test/h2o.jl:5 and test/h2o_st1.jl:5
+
+
+@pytest.fixture(scope="module")
+def h2o():
+    return read_fcidump(H2O)
+
+
+@pytest.fixture(scope="module")
+def h2o_st1():
+    return read_fcidump(H2O_ST1)
+
+
+# ---------------------------------------------------------------- reference energies
+def test_h2o_ccsd_t(h2o):
+    """test/h2o.jl:6-8,19-22."""
+    e = ccdriver(ECInfo.from_fcidump(h2o), "ccsd(t)")
+    assert abs(e["HF"] - (-75.6457645933)) < EPS_REF
+    assert abs(e["MP2c"] - (-0.287815830908)) < EPS_REF
+    assert abs(e["CCSD(T)c"] - (-0.329259440500)) < EPS_REF
+
+
+def test_h2o_lambda_ccsd_t(h2o):
+    """test/h2o.jl:9,24-25."""
+    e = ccdriver(ECInfo.from_fcidump(h2o), "λccsd(t)")
+    assert abs(e["ΛCCSD(T)c"] - (-0.326915143863)) < EPS_REF
+
+
+def test_h2o_dcsd(h2o):
+    """test/h2o.jl:10,27-28."""
+    e = ccdriver(ECInfo.from_fcidump(h2o), "dcsd")
+    assert abs(e["DCSDc"] - (-0.328754956597)) < EPS_REF
+
+
+@pytest.mark.parametrize("method,eref", [("ccsd", -0.082041632192), ("dcsd", -0.082498102641)])
+def test_h2o_st1(h2o_st1, method, eref):
+    """Non-Hermitian FCIDUMP, test/h2o_st1.jl:6-9,16-22."""
+    e = ccdriver(ECInfo.from_fcidump(h2o_st1), method)
+    assert abs(e["HF"] - (-76.298014304953)) < EPS_REF
+    assert abs(e["MP2c"] - (-0.069545740864)) < EPS_REF
+    assert abs(e[method.upper() + "c"] - eref) < EPS_REF
+
+
+# ---------------------------------------------------------------- closed forms
+def _fci_two_electron(h, eri):
+    """Singlet FCI for two electrons: Psi = sum_pq C_pq phi_p(1) phi_q(2), C symmetric.
+
+    (H C)_rs = sum_p h_rp C_ps + sum_q h_sq C_rq + sum_pq (rp|sq) C_pq.
+    """
+    n = h.shape[0]
+    I = np.eye(n)
+    H = (np.einsum('rp,sq->rspq', h, I) + np.einsum('sq,rp->rspq', h, I)
+         + np.einsum('rpsq->rspq', eri)).reshape(n * n, n * n)
+    iu = np.triu_indices(n)
+    basis = []
+    for p, q in zip(*iu):
+        c = np.zeros((n, n))
+        c[p, q] = c[q, p] = 1.0
+        basis.append(c.ravel() / np.linalg.norm(c))
+    S = np.array(basis).T
+    Hs = S.T @ H @ S
+    return np.linalg.eigvalsh(0.5 * (Hs + Hs.T))[0]
+
+
+@pytest.mark.parametrize("method", ["ccsd", "dcsd"])
+def test_two_electron_equals_fci(method):
+    """CCSD and DCSD are exact for two electrons (DCSD: Kats & Manby, JCP 139, 021102)."""
+    s = make_df_system(1, 9, 40, 5)
+    EC = ECInfo.from_df(s)
+    EC.options["thr"] = 1e-20
+    EC.options["maxit"] = 200
+    e = ccdriver(EC, method)
+    eri = np.einsum('pqQ,rsQ->pqrs', s.B, s.B)
+    efci = _fci_two_electron(s.h, eri) + s.e0
+    assert abs(e[method.upper()] - efci) < 1e-10
+
+
+def test_two_electron_triples_vanish():
+    """o=1: every triplet has i=j=k and is skipped (cc_triples.jl:66-69), so E(T)=0."""
+    s = make_df_system(1, 6, 20, 2)
+    e = ccdriver(ECInfo.from_df(s), "ccsd(t)")
+    assert e["(T)"] == 0.0 and e["[T]"] == 0.0
+
+
+# ---------------------------------------------------------------- brute force [T]
+def test_bracket_t_bruteforce():
+    """[T] from an explicit T3 over all (i,j,k), as in cc_tests.jl:54-75.
+
+    The restricted i<=j<=k loop with factors 2/1 must equal the unrestricted
+    sum  -sum W (4/3 T3 - 2 T3[a,i,b,k,c,j] + 2/3 T3[c,i,a,j,b,k]),
+    W = T3 (e_a+e_b+e_c-e_i-e_j-e_k), T3 = K/(e_i+e_j+e_k-e_a-e_b-e_c).
+    """
+    s = make_df_system(3, 6, 25, 7)
+    EC = ECInfo.from_df(s)
+    e = ccdriver(EC, "ccsd(t)")
+    T2 = EC.files["T_vvoo"]
+    eo, ev = EC.files["e_m"][:3], EC.files["e_m"][3:]
+    vvvo, ovoo = EC.ints2("vvvo"), EC.ints2("ovoo")
+    no, nv = 3, 6
+    T3 = np.zeros((nv, no, nv, no, nv, no))
+    for i in range(no):
+        for j in range(no):
+            for k in range(no):
+                K = triples._kijk(T2, vvvo, ovoo, i, j, k)
+                D = eo[i] + eo[j] + eo[k] - ev[:, None, None] - ev[None, :, None] - ev[None, None, :]
+                T3[:, i, :, j, :, k] = K / D
+    Dv = ev[:, None, None, None, None, None] + ev[None, None, :, None, None, None] + ev[None, None, None, None, :, None]
+    Do = eo[None, :, None, None, None, None] + eo[None, None, None, :, None, None] + eo[None, None, None, None, None, :]
+    W = T3 * (Dv - Do)
+    brute = -np.sum(W * (4.0 / 3.0 * T3 - 2.0 * T3.transpose(0, 1, 2, 5, 4, 3)
+                         + 2.0 / 3.0 * T3.transpose(2, 1, 4, 3, 0, 5)))
+    assert abs(brute - e["[T]"]) < 1e-12 * max(1.0, abs(brute))
+
+
+# ---------------------------------------------------------------- Lambda residual = dL/dT
+def _converged(no, nv, nq, seed, method="ccsd"):
+    s = make_df_system(no, nv, nq, seed)
+    EC = ECInfo.from_df(s)
+    EC.options["thr"] = 1e-22
+    EC.options["maxit"] = 300
+    ccdriver(EC, method)
+    return s, EC
+
+
+@pytest.mark.parametrize("dc", [False, True])
+def test_lambda_residual_is_lagrangian_gradient(dc):
+    """R_Lambda(U) = d/dT [E(T) + U1.R1(T) + U2.R2(T)] (equations.tex:755-801)."""
+    s, EC = _converged(3, 7, 30, 11, "dcsd" if dc else "ccsd")
+    T1 = EC.files["T_vo"].copy()
+    T2 = EC.files["T_vvoo"].copy()
+    rng = np.random.default_rng(3)
+    U1 = 0.01 * rng.standard_normal(T1.shape)
+    U2 = 0.01 * rng.standard_normal(T2.shape)
+    U2 = U2 + U2.transpose(1, 0, 3, 2)
+    calc_dressed_ints(EC, T1, calc_d_vovv=True)
+    lagrange.calc_3ext_times_T2(EC, T2)
+    RL1, RL2 = lagrange.calc_ccsd_vector_times_Jacobian(EC, U1, U2, dc=dc)
+    dT1 = rng.standard_normal(T1.shape)
+    dT2 = rng.standard_normal(T2.shape)
+    dT2 = dT2 + dT2.transpose(1, 0, 3, 2)
+
+    def L(t1, t2):
+        files = dict(EC.files)
+        R1, R2 = calc_cc_resid(EC, t1, t2, dc=dc)
+        EC.files.update(files)
+        E = calc_doubles_energy(EC, t2)["E"] + calc_singles_energy(EC, t1)["E"]
+        return E + np.sum(U1 * R1) + np.sum(U2 * R2)
+
+    h = 1e-4
+    fd = lambda h: (L(T1 + h * dT1, T2 + h * dT2) - L(T1 - h * dT1, T2 - h * dT2)) / (2 * h)
+    rich = (4.0 * fd(h / 2) - fd(h)) / 3.0
+    an = np.sum(RL1 * dT1) + np.sum(RL2 * dT2)
+    assert abs(rich - an) < 1e-8 * abs(an)
+
+
+def test_one_rdm_is_energy_derivative():
+    """dE_corr/dh[p,q] = dD1[q,p] (cc_lagrange.jl:793-837), by Richardson-extrapolated FD."""
+    s = make_df_system(3, 7, 30, 11)
+
+    def solve(h1):
+        EC = ECInfo(DFInts(s.B), h1, 0.0, s.nocc)
+        EC.options["thr"] = 1e-22
+        EC.options["maxit"] = 300
+        return ccdriver(EC, "ccsd"), EC
+
+    _, EC = solve(s.h)
+    lagrange.calc_lm_cc(EC)
+    F = EC.files
+    D1, dD1 = lagrange.calc_1RDM(EC, F["U_vo"], F["U_vvoo"], F["T_vo"], F["T_vvoo"])
+    H = np.random.default_rng(1).standard_normal(s.h.shape)
+    fd = lambda h: (solve(s.h + h * H)[0]["CCSDc"] - solve(s.h - h * H)[0]["CCSDc"]) / (2 * h)
+    rich = (4.0 * fd(5e-4) - fd(1e-3)) / 3.0
+    an = np.sum(dD1.T * H)
+    assert abs(rich - an) < 1e-7 * abs(an)
+
+
+# ---------------------------------------------------------------- synthetic systems, FCIDUMP
+def test_splitmix64_published_vectors():
+    """First outputs of Vigna's splitmix64 with state 0."""
+    g = np.uint64(0x9E3779B97F4A7C15)
+    x = np.array([0, g, (2 * int(g)) % 2**64], dtype=np.uint64)
+    out = [int(v) for v in splitmix64(x)]
+    assert out == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_synthetic_fock_is_diagonal():
+    s = make_df_system(4, 11, 30, 0)
+    EC = ECInfo.from_df(s)
+    calc_fock_matrix(EC)
+    assert np.max(np.abs(EC.files["f_mm"] - np.diag(s.eps))) < 1e-13
+    assert np.all(np.diff(s.eps[:4]) >= 0) and np.all(np.diff(s.eps[4:]) >= 0)
+    assert np.allclose(s.B, s.B.transpose(1, 0, 2))
+
+
+def test_fcidump_roundtrip(tmp_path):
+    """Config-0 style FCIDUMP: the 4-index path reproduces the DF path."""
+    s = make_df_system(2, 5, 12, 0)
+    p = str(tmp_path / "sys.FCIDUMP")
+    write_fcidump(p, s)
+    fd = read_fcidump(p)
+    e4 = ccdriver(ECInfo.from_fcidump(fd), "ccsd(t)")
+    edf = ccdriver(ECInfo.from_df(s), "ccsd(t)")
+    assert abs(e4["HF"] - edf["HF"]) < 1e-12
+    assert abs(e4["CCSD(T)"] - edf["CCSD(T)"]) < 1e-11
