@@ -1,0 +1,92 @@
+// Closed-shell coupled-cluster context: integrals, dressing, residuals, solver.
+#pragma once
+#include "ecg_tensor.h"
+
+namespace ecg {
+
+// Orbital blocks of an n x n (x naux) quantity, each stored densely.
+struct Blocks {
+  DevBuf oo, ov, vo, vv;
+};
+
+struct SolverOpts {
+  double thr = 1e-10, conven = 0.1, shifts = 0.15, shiftp = 0.2, resthr = 10.0;
+  int maxit = 50, maxdiis = 6;
+  int refine_every = 0;  // mixed precision: FP64 residual every k-th iteration (0 = off)
+};
+
+struct DiisState {
+  int maxdiis = 6, next = 0, ndim = 0;
+  double resthr = 10.0;
+  std::vector<DevBuf> amps, res;
+  std::vector<double> bmat;  // (maxdiis+1)^2, row-major
+};
+
+struct Context {
+  int no = 0, nv = 0, n = 0, nq = 0;
+  int device = 0;
+  bool hermitian = true;     // right DF factor equals the left one
+  double e0 = 0.0;
+  hipStream_t stream = nullptr;
+  TensorOps ops;
+  std::string last_error;
+
+  // undressed inputs
+  DevBuf h;                  // n x n one-electron integrals
+  Blocks L, R;               // DF factors per block, [p,q,Q]
+  DevBuf fock, eps;          // closed-shell Fock matrix (n x n), orbital energies (n)
+  DevBuf oovv;               // <kl|cd> = (kc|ld), [k,l,c,d]
+  DevBuf vvoo;               // <ij|ab> reordered to [a,b,i,j]
+  // dressed (per residual call)
+  Blocks Ld, Rd;
+  DevBuf dh, df;             // dressed one-electron integrals and Fock matrix (n x n)
+  DevBuf trace;              // sum_k Rd_oo[k,k,Q]
+  // amplitude state of the device-resident solver
+  DevBuf T1, T2, R1, R2;
+  DiisState diis;
+  double eref_hf = 0.0;
+  long long residual_calls = 0;
+
+  long nvo() const { return (long)nv * no; }
+  long nvvoo() const { return (long)nv * nv * no * no; }
+
+  void init(int nocc, int nvirt, int naux, const double* h_host, const double* BL_host,
+            const double* BR_host, double e0_in, size_t workspace_bytes);
+  void build_fock();
+  void dress(const double* T1d);                       // T1 on device
+  void residual(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d);
+  void ladder(Prec pr, const double* T2d, double* R2d);
+  // energies
+  double hf_energy();
+  void mp2_guess(double* T1d, double* T2d, double* emp2 /*[E,ESS,EOS]*/);
+  void doubles_energy(const double* T2d, double* e /*[E,ESS,EOS]*/);
+  void singles_energy(const double* T1d, bool fock_only, double* e);
+  void hylleraas(const double* T1d, const double* T2d, const double* R1d, const double* R2d,
+                 double* e /*[E,ESS,EOS,pE,pESS,pEOS]*/);
+  // amplitude update T += -R/(D+shift)
+  void update(double* T1d, double* T2d, const double* R1d, const double* R2d, double shifts, double shiftp);
+  // closed-shell norms and dots of the reference (cctools.jl)
+  double doubles_dot(const double* X, const double* Y, double cex);  // sum (X - cex X_ba) Y
+  void doubles_de(const double* X, const double* Y, double& d, double& e);
+  DevBuf dots_part;
+  std::vector<double> dots_host;
+  double singles_dot(const double* X, const double* Y) { return ops.dot_host(nvo(), X, Y); }
+  // DIIS (diis.jl) on device vectors; dots: 0 = closed-shell custom dots, 1 = plain
+  void diis_reset(int maxdiis, double resthr);
+  void diis_perform(double* A1, double* A2, const double* R1d, const double* R2d, int dots);
+  // device-resident CC iterations (cc.jl:2518-2648)
+  int iter = 0;
+  void state_init(const double* T1h, const double* T2h, const SolverOpts& o);
+  // info = [E_H, ESS, EOS, E_proj, dE, NormR, NormT, converged]
+  void state_step(int method, Prec pr, const SolverOpts& o, double* info);
+  int solve(int method, Prec pr, const SolverOpts& o, double* energies, int* converged, FILE* log);
+  // <ab|ij> as [a,b,i,j] (equals vvoo for Hermitian integrals)
+  DevBuf abij_buf;
+  const double* abij() const { return hermitian ? vvoo.d() : abij_buf.d(); }
+  // timing of the last residual call (HIP events on the library stream)
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_residual_ms = 0.0, last_ladder_ms = 0.0;
+  ~Context();
+};
+
+}  // namespace ecg

@@ -1,0 +1,63 @@
+// Tensor-GEMM engine for gfx950: C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * C(m,n).
+//
+// Rows (m) of A and C and columns (n) of B and C may each be a pair of tensor
+// indices with independent strides ("split strides"):
+//     A(m,k) = A[(m % a_d0)*a_s0 + (m / a_d0)*a_s1 + k*a_k]
+//     B(k,n) = B[(n % b_d0)*b_s0 + (n / b_d0)*b_s1 + k*b_k]
+//     C(m,n) = C[(m % c_md0)*c_ms0 + (m / c_md0)*c_ms1 + (n % c_nd0)*c_ns0 + (n / c_nd0)*c_ns1]
+// so permuted outputs (the (T) intermediates, the DF ladder blocks) are written
+// in place without a separate transpose pass.  An optional batch index z adds
+// z*a_z, z*b_z, z*c_z to the three base pointers.
+#pragma once
+#include "ecg_common.h"
+
+namespace ecg {
+
+struct GemmOp {
+  int M = 0, N = 0, K = 0, batch = 1;
+  const double* A = nullptr; long a_s0 = 1, a_s1 = 0, a_k = 0, a_z = 0; int a_d0 = 1 << 30;
+  const double* B = nullptr; long b_s0 = 1, b_s1 = 0, b_k = 0, b_z = 0; int b_d0 = 1 << 30;
+  double* C = nullptr; long c_ms0 = 1, c_ms1 = 0, c_ns0 = 0, c_ns1 = 0, c_z = 0;
+  int c_md0 = 1 << 30, c_nd0 = 1 << 30;
+  double alpha = 1.0, beta = 0.0;
+
+  // Plain column-major helpers: A is M x K (lda), or K x M when transposed.
+  GemmOp& a_colmajor(const double* p, long lda, bool trans) {
+    A = p; a_d0 = 1 << 30; a_s1 = 0;
+    if (!trans) { a_s0 = 1; a_k = lda; } else { a_s0 = lda; a_k = 1; }
+    return *this;
+  }
+  GemmOp& b_colmajor(const double* p, long ldb, bool trans) {
+    B = p; b_d0 = 1 << 30; b_s1 = 0;
+    if (!trans) { b_s0 = ldb; b_k = 1; } else { b_s0 = 1; b_k = ldb; }
+    return *this;
+  }
+  GemmOp& c_colmajor(double* p, long ldc) {
+    C = p; c_md0 = 1 << 30; c_ms0 = 1; c_ms1 = 0; c_nd0 = 1 << 30; c_ns0 = ldc; c_ns1 = 0;
+    return *this;
+  }
+};
+
+// Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,
+// FP32 accumulation), or BF16 hi/lo split (three BF16 MFMA products per pair,
+// FP32 accumulation).  Results are always returned in FP64.
+struct GemmEngine {
+  hipStream_t stream = nullptr;
+  DevBuf ws;          // split-K partial sums
+  Prec prec = Prec::F64;
+  long long flops = 0;   // running count of 2*M*N*K issued (for reporting)
+  // optional per-launch timing with HIP events on `stream` (off by default)
+  bool timed = false;
+  double timed_ms = 0.0;
+  long long timed_launches = 0;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  void collect_timing();   // synchronises and accumulates recorded launches
+  void run(const GemmOp& op);
+  void run_prec(const GemmOp& op, Prec p);
+};
+
+// FP32 / BF16-split products (ecg_gemm_lowp.hip).
+void gemm_lowprec(GemmEngine& eng, const GemmOp& op, Prec p);
+
+}  // namespace ecg

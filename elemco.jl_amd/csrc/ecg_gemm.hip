@@ -1,0 +1,285 @@
+// MFMA GEMM kernels for gfx950 (see ecg_gemm.h for the operand addressing).
+//
+// FP64 path: v_mfma_f64_16x16x4_f64.  A 256-thread workgroup (4 waves, 2x2)
+// computes a 128x128 tile of C; each wave owns 64x64 = 4x4 MFMA tiles held in
+// registers.  K advances in slabs of 16 staged through double-buffered LDS
+// (register prefetch of the next slab overlaps the MFMAs of the current one).
+// MFMA operand layout of the 16x16x4 f64 builtin on gfx950 (checked with a lane probe):
+//   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16*(i%4), slot i/4.
+#include "ecg_gemm.h"
+
+#include <algorithm>
+
+namespace ecg {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+struct GemmParams {
+  int M, N, K, kchunk, ksplit, tiles_m, tiles_n;
+  const double* A; long a_s0, a_s1, a_k, a_z; int a_d0;
+  const double* B; long b_s0, b_s1, b_k, b_z; int b_d0;
+  double* C; long c_ms0, c_ms1, c_ns0, c_ns1, c_z; int c_md0, c_nd0;
+  double alpha, beta;
+  int cplain;   // 1: C is a plain column-major partial-sum slab (split-K workspace)
+  long ldc_ws;  // leading dimension of the workspace slab
+};
+
+__device__ __forceinline__ long split_off(int x, int d0, long s0, long s1) {
+  int x1 = x / d0;
+  int x0 = x - x1 * d0;
+  return (long)x0 * s0 + (long)x1 * s1;
+}
+
+// Workgroup id -> (tile_m, tile_n).  Consecutive hardware ids are dealt
+// round-robin to the 8 XCDs; renumber so each XCD receives a contiguous range
+// of logical tiles, then walk tiles in groups of 8 M-tiles so neighbours on
+// one XCD share A and B panels in that XCD's L2.
+__device__ __forceinline__ void tile_coords(int wg, int nwg, int tiles_m, int& tm, int& tn) {
+  const int nx = 8;
+  int q = nwg / nx, r = nwg % nx;
+  int xcd = wg % nx, idx = wg / nx;
+  int logical = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + idx;
+  const int G = 8;
+  int tn_count = nwg / tiles_m;
+  int gid = logical / (G * tn_count);
+  int first_m = gid * G;
+  int gsize = min(tiles_m - first_m, G);
+  int in = logical - gid * G * tn_count;
+  tm = first_m + in % gsize;
+  tn = in / gsize;
+}
+
+constexpr int BM = 128, BN = 128, BKS = 16, LDP = BM + 16;
+
+template <bool AK, bool BK>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmParams p) {
+  __shared__ double As[2][BKS][LDP];
+  __shared__ double Bs[2][BKS][LDP];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
+  const int bm = tm * BM, bn = tn * BN;
+  const int z = blockIdx.z;
+  const int bz = z / p.ksplit, kz = z - bz * p.ksplit;
+  const int kbeg = kz * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const double* A = p.A + (long)bz * p.a_z;
+  const double* B = p.B + (long)bz * p.b_z;
+
+  // per-thread row / column offsets (fixed along K)
+  long aoff[8], boff[8];
+  bool aok[8], bok[8];
+  if (!AK) {
+    int m = bm + (t & 127);
+    bool ok = m < p.M;
+    long o = ok ? split_off(m, p.a_d0, p.a_s0, p.a_s1) : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) { aoff[j] = o; aok[j] = ok; }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int m = bm + (t >> 4) + 16 * j;
+      aok[j] = m < p.M;
+      aoff[j] = aok[j] ? split_off(m, p.a_d0, p.a_s0, p.a_s1) : 0;
+    }
+  }
+  if (!BK) {
+    int n = bn + (t & 127);
+    bool ok = n < p.N;
+    long o = ok ? split_off(n, p.b_d0, p.b_s0, p.b_s1) : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) { boff[j] = o; bok[j] = ok; }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int n = bn + (t >> 4) + 16 * j;
+      bok[j] = n < p.N;
+      boff[j] = bok[j] ? split_off(n, p.b_d0, p.b_s0, p.b_s1) : 0;
+    }
+  }
+
+  double ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int k = AK ? (k0 + (t & 15)) : (k0 + (t >> 7) + 2 * j);
+      ra[j] = (aok[j] && k < kend) ? A[aoff[j] + (long)k * p.a_k] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int k = BK ? (k0 + (t & 15)) : (k0 + (t >> 7) + 2 * j);
+      rb[j] = (bok[j] && k < kend) ? B[boff[j] + (long)k * p.b_k] : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (AK) As[buf][t & 15][(t >> 4) + 16 * j] = ra[j];
+      else    As[buf][(t >> 7) + 2 * j][t & 127] = ra[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (BK) Bs[buf][t & 15][(t >> 4) + 16 * j] = rb[j];
+      else    Bs[buf][(t >> 7) + 2 * j][t & 127] = rb[j];
+    }
+  };
+
+  d4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  const int li = lane & 15, lk = lane >> 4;
+  int cur = 0;
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += BKS) {
+    const bool more = k0 + BKS < kend;
+    if (more) gload(k0 + BKS);
+#pragma unroll
+    for (int kk = 0; kk < BKS; kk += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[cur][kk + lk][wm * 64 + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[cur][kk + lk][wn * 64 + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) sstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  // epilogue
+  double* C = p.C + (long)z * p.c_z;  // z-slab for split-K workspaces, batch for direct output
+  if (!p.cplain) C = p.C + (long)bz * p.c_z;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int n = bn + wn * 64 + j * 16 + li;
+    if (n >= p.N) continue;
+    long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = bm + wm * 64 + i * 16 + lk + 4 * r;
+        if (m >= p.M) continue;
+        long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
+        double v = p.alpha * acc[i][j][r];
+        if (p.beta != 0.0) v += p.beta * C[cm + cn];
+        C[cm + cn] = v;
+      }
+    }
+  }
+}
+
+// C = alpha * sum_s W[s] + beta * C over split-K partial slabs W[s] (M x N plain).
+__global__ void splitk_reduce_kernel(const double* __restrict__ W, int ksplit, long slab, int M, int N,
+                                     double* C, long c_ms0, long c_ms1, int c_md0, long c_ns0,
+                                     long c_ns1, int c_nd0, long c_z, double alpha, double beta) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int bz = blockIdx.y;
+  long total = (long)M * N;
+  if (idx >= total) return;
+  int m = (int)(idx % M), n = (int)(idx / M);
+  const double* w = W + (long)bz * ksplit * slab + idx;
+  double s = 0.0;
+  for (int q = 0; q < ksplit; ++q) s += w[(long)q * slab];
+  double* c = C + (long)bz * c_z + split_off(m, c_md0, c_ms0, c_ms1) + split_off(n, c_nd0, c_ns0, c_ns1);
+  double v = alpha * s;
+  if (beta != 0.0) v += beta * (*c);
+  *c = v;
+}
+
+void GemmEngine::run(const GemmOp& op) { run_prec(op, prec); }
+
+void GemmEngine::collect_timing() {
+  for (size_t k = 0; k + 1 < evused; k += 2) {
+    ECG_HIP(hipEventSynchronize(evpool[k + 1]));
+    float ms = 0.f;
+    ECG_HIP(hipEventElapsedTime(&ms, evpool[k], evpool[k + 1]));
+    timed_ms += ms;
+    timed_launches++;
+  }
+  evused = 0;
+}
+
+void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
+  if (op.M <= 0 || op.N <= 0 || op.batch <= 0) return;
+  ECG_CHECK(op.K > 0, "gemm: K must be positive");
+  if (pr != Prec::F64) {
+    gemm_lowprec(*this, op, pr);
+    return;
+  }
+  flops += 2LL * op.M * op.N * (long long)op.K * op.batch;
+  GemmParams p{};
+  p.M = op.M; p.N = op.N; p.K = op.K;
+  p.tiles_m = (op.M + BM - 1) / BM;
+  p.tiles_n = (op.N + BN - 1) / BN;
+  p.A = op.A; p.a_s0 = op.a_s0; p.a_s1 = op.a_s1; p.a_k = op.a_k; p.a_z = op.a_z; p.a_d0 = op.a_d0;
+  p.B = op.B; p.b_s0 = op.b_s0; p.b_s1 = op.b_s1; p.b_k = op.b_k; p.b_z = op.b_z; p.b_d0 = op.b_d0;
+  p.alpha = op.alpha; p.beta = op.beta;
+  // split K when the output grid alone cannot fill 256 CUs
+  long tiles = (long)p.tiles_m * p.tiles_n * op.batch;
+  int ksplit = 1;
+  if (tiles < 512 && op.K >= 4 * BKS * 8) {
+    ksplit = (int)std::min<long>((1024 + tiles - 1) / tiles, op.K / (BKS * 8));
+    ksplit = std::max(1, std::min(ksplit, 64));
+  }
+  int kchunk = (op.K + ksplit - 1) / ksplit;
+  kchunk = ((kchunk + BKS - 1) / BKS) * BKS;
+  ksplit = (op.K + kchunk - 1) / kchunk;
+  p.kchunk = kchunk; p.ksplit = ksplit;
+  long slab = (long)op.M * op.N;
+  if (ksplit > 1) {
+    ws.ensure(sizeof(double) * slab * ksplit * op.batch);
+    p.C = ws.d(); p.cplain = 1; p.ldc_ws = op.M; p.c_z = slab;
+  } else {
+    p.C = op.C; p.cplain = 0;
+    p.c_ms0 = op.c_ms0; p.c_ms1 = op.c_ms1; p.c_md0 = op.c_md0;
+    p.c_ns0 = op.c_ns0; p.c_ns1 = op.c_ns1; p.c_nd0 = op.c_nd0; p.c_z = op.c_z;
+  }
+  if (ksplit > 1) { p.alpha = 1.0; p.beta = 0.0; }
+  dim3 grid(p.tiles_m * p.tiles_n, 1, op.batch * ksplit);
+  ECG_CHECK(grid.z <= 65535, "gemm: batch*ksplit exceeds grid limit");
+  const bool ak = (op.a_k == 1 && op.a_s0 != 1);
+  const bool bk = (op.b_k == 1 && op.b_s0 != 1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    while (evpool.size() < evused + 2) {
+      hipEvent_t e;
+      ECG_HIP(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    e0 = evpool[evused++];
+    e1 = evpool[evused++];
+    ECG_HIP(hipEventRecord(e0, stream));
+  }
+  if (!ak && !bk) gemm_f64_kernel<false, false><<<grid, dim3(256), 0, stream>>>(p);
+  else if (ak && !bk) gemm_f64_kernel<true, false><<<grid, dim3(256), 0, stream>>>(p);
+  else if (!ak && bk) gemm_f64_kernel<false, true><<<grid, dim3(256), 0, stream>>>(p);
+  else gemm_f64_kernel<true, true><<<grid, dim3(256), 0, stream>>>(p);
+  ECG_HIP(hipGetLastError());
+  if (timed) ECG_HIP(hipEventRecord(e1, stream));
+  if (ksplit > 1) {
+    long total = slab;
+    dim3 g2((unsigned)((total + 255) / 256), op.batch);
+    hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, ws.d(), ksplit, slab, op.M, op.N,
+                       op.C, op.c_ms0, op.c_ms1, op.c_md0, op.c_ns0, op.c_ns1, op.c_nd0, op.c_z,
+                       op.alpha, op.beta);
+    ECG_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace ecg

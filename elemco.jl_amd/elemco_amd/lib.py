@@ -1,0 +1,233 @@
+"""ctypes binding of libelemco_hip.so (declarations: include/elemco_hip.h).
+
+The library is the product path; importing this module fails loudly when the
+shared object is missing or cannot be loaded -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("ELEMCO_HIP_LIB", os.path.join(os.path.dirname(_HERE), "libelemco_hip.so"))
+
+ECG_CCSD, ECG_DCSD = 0, 1
+ECG_FP64, ECG_FP32, ECG_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": ECG_FP64, "fp32": ECG_FP32, "bf16x3": ECG_BF16X3}
+
+
+class ElemcoError(RuntimeError):
+    """Error reported by the HIP library (status < 0 and ecg_last_error())."""
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("thr", ctypes.c_double), ("conven", ctypes.c_double), ("shifts", ctypes.c_double),
+                ("shiftp", ctypes.c_double), ("resthr", ctypes.c_double), ("maxit", ctypes.c_int),
+                ("maxdiis", ctypes.c_int), ("refine_every", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "ecg_default_opts": [ctypes.POINTER(SolverOpts)],
+    "ecg_create": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                   ctypes.c_double, ctypes.c_int, ctypes.c_size_t],
+    "ecg_destroy": [_vp],
+    "ecg_last_error": [_vp],
+    "ecg_fock": [_vp, _dp, _dp, _dp],
+    "ecg_mp2": [_vp, _dp, _dp, _dp],
+    "ecg_cc_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp],
+    "ecg_cc_residual_dev": [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp],
+    "ecg_cc_state_init": [_vp, _dp, _dp, ctypes.POINTER(SolverOpts)],
+    "ecg_cc_state_step": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp],
+    "ecg_cc_state_get": [_vp, _dp, _dp],
+    "ecg_cc_solve": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _ip, _ip],
+    "ecg_pert_triples": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_lambda_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
+    "ecg_one_rdm": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_stats": [_vp, _dp],
+}
+_RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p}
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def load():
+    """Load the shared library once; raises ElemcoError if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ElemcoError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = _RESTYPES.get(name, ctypes.c_int)
+    _lib = lib
+    return lib
+
+
+def default_opts(**kw) -> SolverOpts:
+    o = SolverOpts()
+    load().ecg_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown solver option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def fortran(a, shape=None):
+    """Column-major FP64 copy/view of ``a`` (the Julia layout the C ABI expects)."""
+    a = np.asarray(a, dtype=np.float64)
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
+    return np.asfortranarray(a)
+
+
+class Context:
+    """One library context: integrals resident on one GPU (``ecg_create``)."""
+
+    def __init__(self, nocc, nvirt, naux, h, BL, BR=None, e0=0.0, device=0, workspace_bytes=0):
+        lib = load()
+        n = nocc + nvirt
+        self.nocc, self.nvirt, self.naux, self.norb = nocc, nvirt, naux, n
+        h = fortran(h, (n, n))
+        BL = fortran(BL, (n, n, naux))
+        BR = None if BR is None else fortran(BR, (n, n, naux))
+        handle = _vp()
+        st = lib.ecg_create(ctypes.byref(handle), nocc, nvirt, naux, _ptr(h), _ptr(BL), _ptr(BR), float(e0),
+                            int(device), int(workspace_bytes))
+        if st != 0:
+            raise ElemcoError(f"ecg_create failed ({st}): {lib.ecg_last_error(None).decode()}")
+        self._h = handle
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ecg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st != 0:
+            raise ElemcoError(self._lib.ecg_last_error(self._h).decode())
+
+    # ---- shapes
+    def zeros1(self):
+        return np.zeros((self.nvirt, self.nocc), order="F")
+
+    def zeros2(self):
+        return np.zeros((self.nvirt, self.nvirt, self.nocc, self.nocc), order="F")
+
+    def _t1(self, a):
+        return fortran(a, (self.nvirt, self.nocc))
+
+    def _t2(self, a):
+        return fortran(a, (self.nvirt, self.nvirt, self.nocc, self.nocc))
+
+    # ---- entry points
+    def fock(self):
+        n = self.norb
+        f = np.zeros((n, n), order="F")
+        e = np.zeros(n)
+        ehf = ctypes.c_double()
+        self._chk(self._lib.ecg_fock(self._h, _ptr(f), _ptr(e), ctypes.byref(ehf)))
+        return f, e, ehf.value
+
+    def mp2(self):
+        T1, T2, e = self.zeros1(), self.zeros2(), np.zeros(3)
+        self._chk(self._lib.ecg_mp2(self._h, _ptr(T1), _ptr(T2), _ptr(e)))
+        return T1, T2, e
+
+    def cc_residual(self, T1, T2, method=ECG_CCSD, precision=ECG_FP64):
+        T1, T2 = self._t1(T1), self._t2(T2)
+        R1, R2 = self.zeros1(), self.zeros2()
+        self._chk(self._lib.ecg_cc_residual(self._h, method, precision, _ptr(T1), _ptr(T2), _ptr(R1), _ptr(R2)))
+        return R1, R2
+
+    def cc_residual_dev(self, dT1, dT2, dR1, dR2, method=ECG_CCSD, precision=ECG_FP64):
+        """Device-pointer variant: arguments are integer device addresses."""
+        self._chk(self._lib.ecg_cc_residual_dev(self._h, method, precision, dT1, dT2, dR1, dR2))
+
+    def cc_solve(self, method=ECG_CCSD, precision=ECG_FP64, opts=None):
+        opts = opts or default_opts()
+        T1, T2, e = self.zeros1(), self.zeros2(), np.zeros(6)
+        it, conv = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.ecg_cc_solve(self._h, method, precision, ctypes.byref(opts), _ptr(T1), _ptr(T2),
+                                         _ptr(e), ctypes.byref(it), ctypes.byref(conv)))
+        return e, T1, T2, it.value, bool(conv.value)
+
+    def state_init(self, T1=None, T2=None, opts=None):
+        opts = opts or default_opts()
+        if T1 is not None:
+            T1, T2 = self._t1(T1), self._t2(T2)
+        self._chk(self._lib.ecg_cc_state_init(self._h, _ptr(T1), _ptr(T2), ctypes.byref(opts)))
+
+    def state_step(self, method=ECG_CCSD, precision=ECG_FP64, opts=None):
+        opts = opts or default_opts()
+        info = np.zeros(8)
+        self._chk(self._lib.ecg_cc_state_step(self._h, method, precision, ctypes.byref(opts), _ptr(info)))
+        return info
+
+    def state_get(self):
+        T1, T2 = self.zeros1(), self.zeros2()
+        self._chk(self._lib.ecg_cc_state_get(self._h, _ptr(T1), _ptr(T2)))
+        return T1, T2
+
+    def pert_triples(self, T1, T2, U1=None, U2=None, precision=ECG_FP64):
+        lam = U1 is not None
+        T1, T2 = self._t1(T1), self._t2(T2)
+        if lam:
+            U1, U2 = self._t1(U1), self._t2(U2)
+        et, etb = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._lib.ecg_pert_triples(self._h, int(lam), precision, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2),
+                                             ctypes.byref(et), ctypes.byref(etb)))
+        return et.value, etb.value
+
+    def lambda_residual(self, T1, T2, U1, U2, method=ECG_CCSD, precision=ECG_FP64):
+        T1, T2, U1, U2 = self._t1(T1), self._t2(T2), self._t1(U1), self._t2(U2)
+        R1, R2 = self.zeros1(), self.zeros2()
+        self._chk(self._lib.ecg_lambda_residual(self._h, method, precision, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2),
+                                                _ptr(R1), _ptr(R2)))
+        return R1, R2
+
+    def lambda_solve(self, T1, T2, method=ECG_CCSD, opts=None):
+        opts = opts or default_opts()
+        T1, T2 = self._t1(T1), self._t2(T2)
+        U1, U2 = self.zeros1(), self.zeros2()
+        it, conv = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.ecg_lambda_solve(self._h, method, ctypes.byref(opts), _ptr(T1), _ptr(T2), _ptr(U1),
+                                             _ptr(U2), ctypes.byref(it), ctypes.byref(conv)))
+        return U1, U2, it.value, bool(conv.value)
+
+    def one_rdm(self, T1, T2, U1, U2):
+        T1, T2, U1, U2 = self._t1(T1), self._t2(T2), self._t1(U1), self._t2(U2)
+        n = self.norb
+        D1, dD1 = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        self._chk(self._lib.ecg_one_rdm(self._h, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2), _ptr(D1), _ptr(dD1)))
+        return D1, dD1
+
+    def stats(self):
+        s = np.zeros(5)
+        self._chk(self._lib.ecg_stats(self._h, _ptr(s)))
+        return dict(residual_ms=s[0], ladder_ms=s[1], gemm_ms=s[2], gemm_launches=int(s[3]), gemm_flops=s[4])

@@ -1,0 +1,60 @@
+"""Seeded synthetic density-fitted closed-shell systems (the benchmark inputs).
+
+A system is fully determined by ``(nocc, nvirt, naux, seed)``:
+occupied orbital energies ~ U[-1.5, -0.3] Eh and virtual ~ U[0.2, 3.0] Eh
+(each block sorted ascending), symmetric DF factors ``B[p,q,Q] ~ N(0, 0.05^2)``
+(upper triangle drawn in the order Q, q, p<=q and mirrored), Fock matrix
+``diag(eps)`` and one-electron integrals ``h = F - (2J - K)`` over the occupied
+orbitals, so the closed-shell Fock build reproduces ``diag(eps)``; ``E0 = 0``.
+Random numbers: splitmix64 of a (seed, stream, counter) key, uniform doubles
+from the top 53 bits, normals by Box-Muller -- reproducible on any machine.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_MASK = (1 << 64) - 1
+
+
+def _mix(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _draws(seed, stream, count):
+    with np.errstate(over="ignore"):
+        base = _mix(np.array([(seed * 0x100000001B3 + stream) & _MASK], dtype=np.uint64))[0]
+        ctr = np.arange(count, dtype=np.uint64)
+        return _mix(base ^ (ctr * np.uint64(0xD1B54A32D192ED03)))
+
+
+def _uniform(seed, stream, count):
+    return (_draws(seed, stream, count) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _normal(seed, stream, count):
+    u = _uniform(seed, stream, 2 * count)
+    return np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
+
+
+def make_system(nocc: int, nvirt: int, naux: int, seed: int):
+    """Return (eps, h, B, e0) with B of shape (n, n, naux) in column-major order."""
+    n = nocc + nvirt
+    eps = np.concatenate([np.sort(-1.5 + 1.2 * _uniform(seed, 1, nocc)),
+                          np.sort(0.2 + 2.8 * _uniform(seed, 2, nvirt))])
+    ntri = n * (n + 1) // 2
+    g = 0.05 * _normal(seed, 3, ntri * naux).reshape(naux, ntri)
+    rows, cols = np.triu_indices(n)
+    tri = rows + cols * (cols + 1) // 2
+    B = np.empty((n, n, naux), order="F")
+    for Q in range(naux):
+        m = np.zeros((n, n))
+        m[rows, cols] = g[Q, tri]
+        B[:, :, Q] = m + np.triu(m, 1).T
+    Bo = B[:, :nocc, :]
+    J = np.tensordot(B, np.einsum('iiQ->Q', B[:nocc, :nocc, :]), axes=([2], [0]))
+    K = np.einsum('piQ,iqQ->pq', Bo, B[:nocc, :, :])
+    h = np.diag(eps) - (2.0 * J - K)
+    return eps, np.asfortranarray(h), B, 0.0

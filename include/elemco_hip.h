@@ -1,0 +1,115 @@
+/* elemco_hip.h -- C ABI of the MI355X (gfx950) closed-shell coupled-cluster library.
+ *
+ * Drop-in replacement for the closed-shell hot path of ElemCo.jl v0.14.0
+ * (file:line citations are relative to that source tree).  All arrays are
+ * column-major (Julia layout, first index fastest) and FP64 unless stated.
+ * Host pointers are owned by the caller; the context owns all device memory.
+ * Every call is synchronous (the library's HIP stream is drained on return).
+ * Status: 0 = success, negative = error; ecg_last_error() returns the message
+ * (the Julia binding raises it with error(), the reference's convention).
+ * Orbital spaces: occupied = first nocc orbitals, virtual = the next nvirt
+ * (setup_space_fd!, src/infos/ecinfos.jl:171-192); n = nocc + nvirt.
+ * Integrals are density-fitted: (pq|rs) = sum_Q BL[p,q,Q] * BR[r,s,Q]
+ * (BR = NULL means BR = BL, the Hermitian case; layout of mmL[p,q,L] in
+ * src/integrals/dftools.jl:102).  A four-index FCIDUMP is brought to this form
+ * by an exact factorisation of its (pq|rs) supermatrix (host side).
+ */
+#ifndef ELEMCO_HIP_H
+#define ELEMCO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ecg_ctx ecg_ctx;
+
+enum { ECG_CCSD = 0, ECG_DCSD = 1 };
+enum { ECG_FP64 = 0, ECG_FP32 = 1, ECG_BF16X3 = 2 };
+
+/* Iteration options; defaults of CcOptions/DiisOptions (src/infos/options.jl:128-221,272-282). */
+typedef struct {
+  double thr;        /* squared residual norm threshold (1e-10) */
+  double conven;     /* energy threshold factor: |dE| < sqrt(thr)*conven (0.1) */
+  double shifts;     /* singles level shift (0.15) */
+  double shiftp;     /* doubles level shift (0.2) */
+  double resthr;     /* DIIS start threshold (10.0) */
+  int maxit;         /* maximum iterations (50) */
+  int maxdiis;       /* DIIS subspace (6) */
+  int refine_every;  /* mixed precision: FP64 residual every k-th iteration (0: never) */
+} ecg_solver_opts;
+
+void ecg_default_opts(ecg_solver_opts* opts);
+
+/* Create a context on HIP device `device`.  h: n x n one-electron integrals,
+ * BL (and BR or NULL): n x n x naux DF factors, e0: core energy.
+ * workspace_bytes = 0 picks a size from (nocc, nvirt, naux).
+ * Replaces the FCIDUMP set-up of ccdriver (src/cc/ccdriver.jl:37-76, read_fcidump
+ * src/integrals/dump.jl:326) for the closed-shell path. */
+int ecg_create(ecg_ctx** ctx, int nocc, int nvirt, int naux, const double* h, const double* BL,
+               const double* BR, double e0, int device, size_t workspace_bytes);
+int ecg_destroy(ecg_ctx* ctx);
+const char* ecg_last_error(const ecg_ctx* ctx);
+
+/* Closed-shell Fock matrix (n x n), orbital energies (n) and HF energy:
+ * gen_fock (src/scf/fockfactory.jl:27-30), calc_fock_matrix / calc_HF_energy
+ * (src/cc/cctools.jl:35-76).  Any output pointer may be NULL. */
+int ecg_fock(ecg_ctx* ctx, double* fock, double* eps, double* ehf);
+
+/* MP2 amplitudes and energy [E, ESS, EOS]: calc_MP2 (src/cc/cc.jl:1011-1024). */
+int ecg_mp2(ecg_ctx* ctx, double* T1, double* T2, double* emp2);
+
+/* Closed-shell CCSD (method=ECG_CCSD) or DCSD (ECG_DCSD) residual for given
+ * amplitudes T1 (v x o), T2 (v x v x o x o): calc_cc_resid(EC, T1, T2; dc)
+ * (src/cc/cc.jl:1343-1489). */
+int ecg_cc_residual(ecg_ctx* ctx, int method, int precision, const double* T1, const double* T2,
+                    double* R1, double* R2);
+/* Same with device pointers (already resident in HBM). */
+int ecg_cc_residual_dev(ecg_ctx* ctx, int method, int precision, const double* dT1, const double* dT2,
+                        double* dR1, double* dR2);
+
+/* Device-resident iterations of cc_iterations! (src/cc/cc.jl:2518-2648).
+ * state_init: amplitudes from T1/T2 (host), or the MP2 guess when both are NULL;
+ * resets DIIS.  state_step: one iteration (residual, Hylleraas energy, update,
+ * DIIS); info = [E_H, ESS, EOS, E_proj, dE, NormR, NormT, converged(0/1)].
+ * state_get: copy the current amplitudes to the host. */
+int ecg_cc_state_init(ecg_ctx* ctx, const double* T1, const double* T2, const ecg_solver_opts* opts);
+int ecg_cc_state_step(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, double* info);
+int ecg_cc_state_get(ecg_ctx* ctx, double* T1, double* T2);
+
+/* calc_cc for the closed-shell path (src/cc/cc.jl:2451-2516): iterate from the
+ * MP2 guess (T1 = T2 = NULL on input) or from the given amplitudes until
+ * converged.  energies = [E_H, ESS, EOS, E_proj, pESS, pEOS]; T1/T2 receive the
+ * converged amplitudes when not NULL. */
+int ecg_cc_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, double* T1,
+                 double* T2, double* energies, int* iterations, int* converged);
+
+/* Perturbative triples for the closed-shell path.  lambda = 0:
+ * calc_pertT_closed_shell (src/cc/cc_triples.jl:39-148) with amplitudes T1, T2;
+ * lambda = 1: calc_LambdapertT_closed_shell (cc_triples.jl:159-288), which also
+ * needs the contravariant multipliers U1, U2.  Returns E(T) and E[T]. */
+int ecg_pert_triples(ecg_ctx* ctx, int lambda, int precision, const double* T1, const double* T2,
+                     const double* U1, const double* U2, double* ET, double* ETb);
+
+/* Lambda residual R = U x Jacobian: calc_ccsd_vector_times_Jacobian(EC, U1, U2; dc)
+ * (src/cc/cc_lagrange.jl:9-212) at amplitudes T1, T2. */
+int ecg_lambda_residual(ecg_ctx* ctx, int method, int precision, const double* T1, const double* T2,
+                        const double* U1, const double* U2, double* R1, double* R2);
+
+/* Lambda iterations lm_cc_iterations! (src/cc/cc_lagrange.jl:1155-1223) at
+ * converged amplitudes T1, T2; U1/U2 receive the multipliers. */
+int ecg_lambda_solve(ecg_ctx* ctx, int method, const ecg_solver_opts* opts, const double* T1,
+                     const double* T2, double* U1, double* U2, int* iterations, int* converged);
+
+/* One-particle density: calc_1RDM (src/cc/cc_lagrange.jl:793-837); D1, dD1 n x n. */
+int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* U1, const double* U2,
+                double* D1, double* dD1);
+
+/* Timing of the last call: [residual_ms, ladder_ms, gemm_ms, gemm_launches, gemm_flops] */
+int ecg_stats(ecg_ctx* ctx, double* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -49,13 +49,19 @@ class DFInts:
         return _es('prQ,qsQ->pqrs', self.B[s1, s3, :], self.B[s2, s4, :])
 
     def kext(self, D2: np.ndarray) -> np.ndarray:
-        n = self.B.shape[0]
+        """``K[p,r,i,j] = sum_{q,s} <pr|qs> D2[q,s,i,j]`` with ``<pr|qs> = sum_Q B[p,q,Q] B[r,s,Q]``.
+
+        Evaluated one p at a time as two matrix products (no four-index
+        transposes): ``V_p[(r,s),q] = B[(r,s),:] B[p,q,:]^T`` then
+        ``K[p] = V_p[r,(s,q)] D2[q,s,(ij)]``.
+        """
+        n, nq = self.B.shape[0], self.B.shape[2]
+        B2 = np.ascontiguousarray(self.B).reshape(n * n, nq)
+        Dt = np.ascontiguousarray(D2.transpose(1, 0, 2, 3)).reshape(n * n, -1)   # [(s,q), ij]
         out = np.empty((n, n) + D2.shape[2:])
-        for p0 in range(0, n, self.batch):
-            p1 = min(n, p0 + self.batch)
-            # <pr|qs> = sum_Q B[p,q,Q] B[r,s,Q]
-            v = _es('pqQ,rsQ->prqs', self.B[p0:p1], self.B)
-            out[p0:p1] = _es('prqs,qsij->prij', v, D2)
+        for p in range(n):
+            Vp = B2 @ self.B[p].T                                  # [(r,s), q]
+            out[p] = (Vp.reshape(n, n * n) @ Dt).reshape((n,) + D2.shape[2:])
         return out
 
 

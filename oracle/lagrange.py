@@ -80,13 +80,16 @@ def _kext_lambda(EC: ECInfo, dU2):
     ints = EC.ints
     if hasattr(ints, 'int2'):
         return _es('pqrs,pqmn->rsmn', ints.int2, dU2)
-    # DF: <pq|rs> = sum_Q B[p,r,Q] B[q,s,Q]
-    out = np.empty((n, n) + dU2.shape[2:])
+    # DF: <pq|rs> = sum_Q B[p,r,Q] B[q,s,Q]; one r at a time as two matrix products:
+    # V_r[(s,q),p] = B[q,s,:] B[p,r,:]^T,  K[r,s,mn] = V_r[s,(q,p)] dU2[p,q,(mn)]
     B = ints.B
-    for r0 in range(0, n, ints.batch):
-        r1 = min(n, r0 + ints.batch)
-        vint = _es('prQ,qsQ->pqrs', B[:, r0:r1], B)
-        out[r0:r1] = _es('pqrs,pqmn->rsmn', vint, dU2)
+    nq = B.shape[2]
+    B2T = np.ascontiguousarray(B.transpose(1, 0, 2)).reshape(n * n, nq)      # [(s,q), Q]
+    dUt = np.ascontiguousarray(dU2.transpose(1, 0, 2, 3)).reshape(n * n, -1)  # [(q,p), mn]
+    out = np.empty((n, n) + dU2.shape[2:])
+    for r in range(n):
+        Vr = B2T @ B[:, r, :].T
+        out[r] = (Vr.reshape(n, n * n) @ dUt).reshape((n,) + dU2.shape[2:])
     return out
 
 

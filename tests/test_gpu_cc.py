@@ -1,0 +1,123 @@
+"""GPU parity of the closed-shell CCSD/DCSD path against the CPU oracle (FP64).
+
+Bars: residuals within 1e-10 max-abs (o=5, v=19) of the oracle; converged
+energies within 1e-9 Eh of the oracle and 1e-6 Eh of the reference's own
+test energies (test/h2o.jl, test/h2o_st1.jl).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from gpu_helpers import requires_gpu
+
+pytestmark = [pytest.mark.gpu, requires_gpu]
+
+import elemco_amd as E  # noqa: E402
+from oracle import ccsd as O  # noqa: E402
+from oracle.synthetic import make_df_system  # noqa: E402
+
+C0 = (5, 19, 72, 0)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    no, nv, nq, seed = C0
+    s = make_df_system(no, nv, nq, seed)
+    EC = E.ECInfo.synthetic(no, nv, nq, seed)
+    ECo = O.ECInfo.from_df(s)
+    O.calc_fock_matrix(ECo)
+    O.calc_MP2(ECo)
+    return s, EC, ECo
+
+
+def test_synthetic_inputs_match_oracle_generator():
+    eps, h, B, e0 = E.make_system(*C0)
+    s = make_df_system(*C0)
+    assert np.array_equal(eps, s.eps) and np.array_equal(B, s.B)
+    assert np.max(np.abs(h - s.h)) < 1e-14
+
+
+def test_fock_hf_mp2(c0):
+    s, EC, ECo = c0
+    f, eps, ehf = EC.ctx.fock()
+    assert np.max(np.abs(f - ECo.files["f_mm"])) < 1e-12
+    assert abs(ehf - O.calc_HF_energy(ECo)) < 1e-11
+    T1, T2, emp2 = EC.ctx.mp2()
+    assert np.max(np.abs(T2 - ECo.files["T_vvoo"])) < 1e-12
+    assert np.max(np.abs(T1 - ECo.files["T_vo"])) < 1e-12
+    e = O.calc_doubles_energy(ECo, ECo.files["T_vvoo"])["E"] + O.calc_singles_energy(ECo, ECo.files["T_vo"], True)["E"]
+    assert abs(emp2[0] - e) < 1e-11
+
+
+@pytest.mark.parametrize("dc", [False, True])
+@pytest.mark.parametrize("amps", ["mp2", "random"])
+def test_residual_c0(c0, dc, amps):
+    s, EC, ECo = c0
+    T1, T2 = ECo.files["T_vo"].copy(), ECo.files["T_vvoo"].copy()
+    if amps == "random":
+        rng = np.random.default_rng(11)
+        T1 = 0.05 * rng.standard_normal(T1.shape)
+        T2 = 0.05 * rng.standard_normal(T2.shape)
+        T2 = T2 + T2.transpose(1, 0, 3, 2)
+    R1o, R2o = O.calc_cc_resid(ECo, T1, T2, dc=dc)
+    R1, R2 = E.calc_cc_resid(EC, T1, T2, dc=dc)
+    assert np.max(np.abs(R1 - R1o)) < 1e-10
+    assert np.max(np.abs(R2 - R2o)) < 1e-10
+
+
+@pytest.mark.parametrize("method", ["ccsd", "dcsd"])
+def test_solve_c0(method):
+    no, nv, nq, seed = C0
+    s = make_df_system(no, nv, nq, seed)
+    eo = O.ccdriver(O.ECInfo.from_df(s), method)
+    EC = E.ECInfo.synthetic(no, nv, nq, seed)
+    e = E.ccdriver(EC, method)
+    assert e["converged"]
+    assert abs(e[method.upper() + "c"] - eo[method.upper() + "c"]) < 1e-9
+    assert abs(e["HF"] - eo["HF"]) < 1e-10
+
+
+def test_h2o_fcidump_ccsd():
+    """Reference fixture test/files/H2O.FCIDUMP through the DF factorisation (test/h2o.jl:6-8)."""
+    EC = E.ECInfo.from_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"))
+    e = E.ccdriver(EC, "ccsd")
+    assert abs(e["HF"] - (-75.6457645933)) < 1e-6
+    assert abs(e["MP2c"] - (-0.287815830908)) < 1e-6
+    from oracle.fcidump import read_fcidump
+    eo = O.ccdriver(O.ECInfo.from_fcidump(read_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"))), "ccsd")
+    assert abs(e["CCSDc"] - eo["CCSDc"]) < 1e-9
+
+
+@pytest.mark.parametrize("method,eref", [("ccsd", -0.082041632192), ("dcsd", -0.082498102641)])
+def test_h2o_st1_fcidump(method, eref):
+    """Non-Hermitian fixture (test/h2o_st1.jl:9): two-sided DF factors."""
+    EC = E.ECInfo.from_fcidump(os.path.join(GOLDEN, "H2O_ST1.FCIDUMP.gz"))
+    e = E.ccdriver(EC, method)
+    assert abs(e["HF"] - (-76.298014304953)) < 1e-6
+    assert abs(e[method.upper() + "c"] - eref) < 1e-6
+
+
+C1_GOLDEN = os.path.join(GOLDEN, "c1_oracle.json")
+
+
+@pytest.mark.skipif(not os.path.exists(C1_GOLDEN), reason="c1 golden vectors not generated")
+@pytest.mark.parametrize("method", ["ccsd", "dcsd"])
+def test_residual_c1_golden(method):
+    """o=20, v=180, naux=600: residual at MP2 amplitudes vs committed oracle values."""
+    g = json.load(open(C1_GOLDEN))
+    c = g["config"]
+    EC = E.ECInfo.synthetic(c["nocc"], c["nvirt"], c["naux"], c["seed"])
+    T1, T2, _ = EC.ctx.mp2()
+    R1, R2 = E.calc_cc_resid(EC, T1, T2, dc=(method == "dcsd"))
+    gr = g[f"resid_mp2_{method}"]
+    R1o = np.array(gr["R1"]).reshape(R1.shape, order="F")
+    assert np.max(np.abs(R1 - R1o)) < 1e-10
+    chk = gr["R2"]
+    assert abs(np.sum(R2 * R2) - chk["sumsq"]) < 1e-10 * max(1.0, chk["sumsq"])
+    w = np.cos(np.arange(R2.size, dtype=np.float64) * 0.001).reshape(R2.shape, order="F")
+    assert abs(np.sum(R2 * w) - chk["weighted"]) < 1e-9
+    for idx, val in chk["samples"]:
+        assert abs(R2[tuple(idx)] - val) < 1e-10

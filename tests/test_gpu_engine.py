@@ -1,0 +1,70 @@
+"""GPU checks of the MFMA GEMM engine and the permutation kernel against numpy (FP64)."""
+import numpy as np
+import pytest
+
+from gpu_helpers import gpu_contract, gpu_permute, requires_gpu
+
+pytestmark = [pytest.mark.gpu, requires_gpu]
+
+# index extents used by the contraction patterns of the CC residual
+SIZES = {"small": dict(a=19, b=19, c=19, d=19, i=5, j=5, k=5, l=5, Q=72, p=24, q=24),
+         "ragged": dict(a=131, b=131, c=131, d=131, i=7, j=7, k=7, l=7, Q=97, p=138, q=138)}
+
+PATTERNS = [
+    ("klcd", "cdij", "klij"),
+    ("klij", "abkl", "abij"),
+    ("klcd", "adkj", "lcaj"),
+    ("lcaj", "cbil", "abij"),
+    ("klcd", "cakl", "ad"),
+    ("klcd", "cdil", "ki"),
+    ("ad", "dbij", "abij"),
+    ("ki", "abkj", "abij"),
+    ("aiQ", "kcQ", "akic"),
+    ("akic", "cbkj", "abij"),
+    ("bkci", "ackj", "abij"),
+    ("acQ", "kiQ", "akci"),
+    ("kcQ", "jiQ", "kjci"),
+    ("kjci", "cakj", "ai"),
+    ("abij", "jb", "ai"),
+    ("ibQ", "bj", "ijQ"),
+    ("pqQ", "Q", "pq"),
+    ("ai", "bj", "abij"),
+]
+
+
+def _rand(labels, ext, rng):
+    return np.asfortranarray(rng.standard_normal(tuple(ext[c] for c in labels)))
+
+
+@pytest.mark.parametrize("size", list(SIZES))
+@pytest.mark.parametrize("la,lb,lc", PATTERNS)
+def test_contract_matches_einsum(size, la, lb, lc):
+    ext = SIZES[size]
+    rng = np.random.default_rng(abs(hash((size, la, lb, lc))) % 2**32)
+    A, B, C = _rand(la, ext, rng), _rand(lb, ext, rng), _rand(lc, ext, rng)
+    ref = 0.7 * np.einsum(f"{la},{lb}->{lc}", A, B) - 0.3 * C
+    got = gpu_contract(0.7, A, la, B, lb, -0.3, C, lc)
+    scale = np.max(np.abs(ref)) + 1e-300
+    assert np.max(np.abs(got - ref)) / scale < 1e-13
+
+
+@pytest.mark.parametrize("shape,perm", [((19, 19, 5, 5), (1, 0, 2, 3)), ((19, 19, 5, 5), (1, 0, 3, 2)),
+                                        ((33, 40, 7, 3), (2, 0, 3, 1)), ((130, 70), (1, 0)),
+                                        ((5, 19, 19, 5), (0, 2, 1, 3))])
+def test_permute(shape, perm):
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal(shape)
+    Y = rng.standard_normal(tuple(shape[p] for p in perm))
+    got = gpu_permute(X, perm, 2.0, 0.5, Y)
+    ref = 2.0 * np.transpose(X, perm) + 0.5 * Y
+    assert np.array_equal(got, ref)
+
+
+def test_gemm_large_split_k():
+    """Small output, long K: exercises the split-K partial sums."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((40, 40, 150, 150))
+    B = rng.standard_normal((150, 150, 40, 40))
+    ref = np.einsum("klcd,cdij->klij", A, B)
+    got = gpu_contract(1.0, A, "klcd", B, "cdij", 0.0, np.zeros((40, 40, 40, 40)), "klij")
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-13
