@@ -232,9 +232,22 @@ int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* 
   });
 }
 
+int ecg_profile(ecg_ctx* ctx, int enable) {
+  return guarded(ctx, [&](ecg::Context& c) {
+    ECG_HIP(hipStreamSynchronize(c.stream));
+    c.ops.gemm.collect_timing();
+    c.ops.gemm.timed = enable != 0;
+    c.ops.gemm.timed_ms = 0.0;
+    c.ops.gemm.timed_launches = 0;
+    c.ops.gemm.flops = 0;
+  });
+}
+
 int ecg_stats(ecg_ctx* ctx, double* stats) {
   return guarded(ctx, [&](ecg::Context& c) {
     ECG_CHECK(stats, "NULL stats");
+    ECG_HIP(hipStreamSynchronize(c.stream));
+    c.ops.gemm.collect_timing();
     stats[0] = c.last_residual_ms;
     stats[1] = c.last_ladder_ms;
     stats[2] = c.ops.gemm.timed_ms;

@@ -275,38 +275,49 @@ void Context::dress(const double* T1d) {
 }
 
 // ---------------------------------------------------------------- particle-particle ladder
-// R2[a,b,i,j] += sum_Q sum_cd L^[a,c,Q] R^[b,d,Q] T2[c,d,i,j]; the integrals
-// V[a,b,c,d] are assembled on MFMA for a block of a at a time and contracted at
-// once, so the full v^4 tensor never exists.
-void Context::ladder(Prec pr, const double* T2d_, double* R2d) {
+// P[a,b,i,j] = sum_Q sum_cd L^[a,c,Q] R^[b,d,Q] T2[c,d,i,j]  is added to R2r, whose
+// final symmetrisation R2 += R2r + R2r[b,a,j,i] restores the full term, because
+// P[a,b,i,j] = P[b,a,j,i] ((ac|bd) = (bd|ac)).  For a block of a the integrals
+// V[a',b',c,d] (b = a0 + b' >= a0) are assembled on MFMA and contracted at once;
+// the v^4 tensor never exists and only b >= a0 is formed (half the work).  Pairs
+// inside the diagonal block are formed twice and weighted by 1/2.
+void Context::ladder(Prec pr, const double* T2x, double* R2r) {
   const Blocks& Rh = hermitian ? Ld : Rd;
   size_t mark = ops.arena.mark();
   size_t free_b = ops.arena.buf.bytes - ops.arena.mark();
-  long v = nv;
-  long per_a = v * v * v;
+  const long v = nv;
+  const long per_a = v * v * v;
   long ab = std::max<long>(1, std::min<long>(v, (long)(free_b * 0.8 / sizeof(double)) / per_a));
   double* V = ops.arena.take(ab * per_a);
+  ops.gemm.stream = stream;
   ECG_HIP(hipEventRecord(ev[2], stream));
   for (long a0 = 0; a0 < v; a0 += ab) {
-    int na = (int)std::min<long>(ab, v - a0);
-    // V[a',b,c,d] = sum_Q L^[a0+a',c,Q] R^[b,d,Q]   (m = (a',c), n = (b,d), k = Q)
+    const long na = std::min<long>(ab, v - a0);
+    const long nb = v - a0;
+    // V[a',b',c,d] = sum_Q L^[a0+a',c,Q] R^[a0+b',d,Q]   (m = (a',c), n = (b',d), k = Q)
     GemmOp g;
-    g.M = (int)(na * v); g.N = (int)(v * v); g.K = nq;
-    g.A = Ld.vv.d() + a0; g.a_s0 = 1; g.a_d0 = na; g.a_s1 = v; g.a_k = v * v;
-    g.B = Rh.vv.d(); g.b_s0 = 1; g.b_d0 = 1 << 30; g.b_s1 = 0; g.b_k = v * v;
-    g.C = V; g.c_ms0 = 1; g.c_md0 = na; g.c_ms1 = (long)na * v;
-    g.c_ns0 = na; g.c_nd0 = (int)v; g.c_ns1 = (long)na * v * v;
+    g.M = (int)(na * v); g.N = (int)(nb * v); g.K = nq;
+    g.A = Ld.vv.d() + a0; g.a_s0 = 1; g.a_d0 = (int)na; g.a_s1 = v; g.a_k = v * v;
+    g.B = Rh.vv.d() + a0; g.b_s0 = 1; g.b_d0 = (int)nb; g.b_s1 = v; g.b_k = v * v;
+    g.C = V; g.c_ms0 = 1; g.c_md0 = (int)na; g.c_ms1 = na * nb;
+    g.c_ns0 = na; g.c_nd0 = (int)nb; g.c_ns1 = na * nb * v;
     g.alpha = 1.0; g.beta = 0.0;
-    ops.gemm.stream = stream;
     ops.gemm.run_prec(g, pr);
-    // R2[a0+a',b,i,j] += sum_{cd} V[a',b,c,d] T[c,d,i,j]   (m = (a',b), n = (i,j), k = (c,d))
-    GemmOp h;
-    h.M = (int)(na * v); h.N = no * no; h.K = (int)(v * v);
-    h.A = V; h.a_s0 = 1; h.a_d0 = na; h.a_s1 = na; h.a_k = (long)na * v;
-    h.B = T2d_; h.b_s0 = v * v; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = 1;
-    h.C = R2d + a0; h.c_ms0 = 1; h.c_md0 = na; h.c_ms1 = v; h.c_ns0 = v * v; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
-    h.alpha = 1.0; h.beta = 1.0;
-    ops.gemm.run_prec(h, pr);
+    // R2r[a0+a',a0+b',i,j] += w sum_{cd} V[a',b',c,d] T[c,d,i,j]  (m = (a',b'), n = (i,j), k = (c,d));
+    // T is supplied as Tx[i,j,c,d] so both operands stream along m / n.
+    for (int part = 0; part < 2; ++part) {
+      const long b0 = part == 0 ? 0 : na;           // diagonal block, then b' >= na
+      const long nbp = part == 0 ? na : nb - na;
+      if (nbp <= 0) continue;
+      GemmOp h;
+      h.M = (int)(na * nbp); h.N = no * no; h.K = (int)(v * v);
+      h.A = V + na * b0; h.a_s0 = 1; h.a_d0 = (int)na; h.a_s1 = na; h.a_k = na * nb;
+      h.B = T2x; h.b_s0 = 1; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = (long)no * no;
+      h.C = R2r + a0 + (a0 + b0) * v; h.c_ms0 = 1; h.c_md0 = (int)na; h.c_ms1 = v;
+      h.c_ns0 = v * v; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
+      h.alpha = part == 0 ? 0.5 : 1.0; h.beta = 1.0;
+      ops.gemm.run_prec(h, pr);
+    }
   }
   ECG_HIP(hipEventRecord(ev[3], stream));
   ops.arena.release_to(mark);
@@ -373,7 +384,6 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   }
   // particle-particle ladder (kext): sum_Q L^[a,i,Q] R^[b,j,Q] + DF ladder
   ops.contract_p(Prec::F64, 1.0, T3d(Ld.vo.d(), v, o, Q), "aiQ", T3d(Rh.vo.d(), v, o, Q), "bjQ", 1.0, tR2, "abij");
-  ladder(pr, T2, R2d);
   double* X = take4();
   if (!dc) {
     // <kl|cd> T[a,d,k,j] T[c,b,i,l]   (cc.jl:1435-1439)
@@ -397,6 +407,16 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   Tensor tR2r = T4d(R2r, v, v, o, o);
   ops.contract_p(pr, 1.0, txad, "ad", tT2, "dbij", 0.0, tR2r, "abij");
   ops.contract_p(pr, -1.0, txki, "ki", tT2, "abkj", 1.0, tR2r, "abij");
+  {
+    // DF particle-particle ladder (half of it; completed by the symmetrisation below)
+    size_t mk = ops.arena.mark();
+    double* T2x = take4();
+    Tensor tT2x = T4d(T2x, o, o, v, v);
+    const int p_x[4] = {2, 3, 0, 1};
+    ops.permute(tT2, p_x, tT2x, 1.0, 0.0);
+    ladder(pr, T2x, R2r);
+    ops.arena.release_to(mk);
+  }
   {
     // Y[a,k,i,c] = d_voov[a,k,i,c] + 1/2 <kl|cd> T~[a,d,i,l];  R2r += Y T~[c,b,k,j]   (cc.jl:1462-1468)
     Tensor tY = T4d(X, v, o, o, v);

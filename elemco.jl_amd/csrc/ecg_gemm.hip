@@ -52,7 +52,7 @@ __device__ __forceinline__ void tile_coords(int wg, int nwg, int tiles_m, int& t
 constexpr int BM = 128, BN = 128, BKS = 16, LDP = BM + 16;
 
 template <bool AK, bool BK>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmParams p) {
+__global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmParams p) {
   __shared__ double As[2][BKS][LDP];
   __shared__ double Bs[2][BKS][LDP];
   const int t = threadIdx.x;
@@ -102,16 +102,21 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmParams p) {
   }
 
   double ra[8], rb[8];
+  // branch-free: out-of-range elements read element 0 (always valid) and are zeroed
   auto gload = [&](int k0) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       int k = AK ? (k0 + (t & 15)) : (k0 + (t >> 7) + 2 * j);
-      ra[j] = (aok[j] && k < kend) ? A[aoff[j] + (long)k * p.a_k] : 0.0;
+      bool ok = aok[j] && k < kend;
+      double x = A[ok ? aoff[j] + (long)k * p.a_k : 0];
+      ra[j] = ok ? x : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       int k = BK ? (k0 + (t & 15)) : (k0 + (t >> 7) + 2 * j);
-      rb[j] = (bok[j] && k < kend) ? B[boff[j] + (long)k * p.b_k] : 0.0;
+      bool ok = bok[j] && k < kend;
+      double x = B[ok ? boff[j] + (long)k * p.b_k : 0];
+      rb[j] = ok ? x : 0.0;
     }
   };
   auto sstore = [&](int buf) {

@@ -324,14 +324,11 @@ void TensorOps::contract_p(Prec pr, double alpha, const Tensor& A, const char* l
         bool bok = b.contig(ng.first) && b.contig(ng.second) && b.contig(kg);
         double cost = 0;
         bool pA = !aok, pB = !bok;
-        if (aok) {
-          bool coal = a.first_str(mg.first) == 1 || a.first_str(kg) == 1;
-          if (!coal) pA = true;
-        }
-        if (bok) {
-          bool coal = b.first_str(ng.first) == 1 || b.first_str(kg) == 1;
-          if (!coal) pB = true;
-        }
+        // the MFMA GEMM streams operands fastest along m (A) and n (B); a
+        // k-contiguous operand would need a transposing LDS store, so it is
+        // transposed in HBM first (a few tenths of a millisecond at these sizes)
+        if (aok && !(a.first_str(mg.first) == 1 || prod_dims(a, M) == 1)) pA = true;
+        if (bok && !(b.first_str(ng.first) == 1 || prod_dims(b, N) == 1)) pB = true;
         if (pA) cost += 2 * sA;
         if (pB) cost += 2 * sB;
         if (tmpC) cost += 3 * sC;
@@ -362,7 +359,7 @@ void TensorOps::contract_p(Prec pr, double alpha, const Tensor& A, const char* l
     ldst = order;
   };
   if (best.permA) make_perm(a, Mo + Ko, Ause, lau);
-  if (best.permB) make_perm(b, Ko + No, Buse, lbu);
+  if (best.permB) make_perm(b, No + Ko, Buse, lbu);
   Lab au{&Ause, lau}, bu{&Buse, lbu};
   auto m1 = best.m.first, m2 = best.m.second, n1 = best.n.first, n2 = best.n.second;
   // the split used for C must be compatible with the one used for A / B

@@ -58,6 +58,42 @@ __global__ void mfma_probe_kernel(const double* a, const double* b, double* out,
 
 extern "C" {
 
+__global__ void fill_pattern_kernel(double* p, long n, double s) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = s * (double)((i * 2654435761L) % 1000 - 500) * 1e-3;
+}
+
+// Time `reps` evaluations of one contraction on device-resident operands; returns ms per call.
+double ecgt_bench_contract(int prec, int na, const long* adims, const char* la, int nb, const long* bdims,
+                           const char* lb, int nc, const long* cdims, const char* lc, int reps) {
+  try {
+    Env& e = env();
+    ecg::Tensor tA = make(nullptr, na, adims), tB = make(nullptr, nb, bdims), tC = make(nullptr, nc, cdims);
+    ecg::DevBuf dA(sizeof(double) * tA.size()), dB(sizeof(double) * tB.size()), dC(sizeof(double) * tC.size());
+    tA.p = dA.d(); tB.p = dB.d(); tC.p = dC.d();
+    hipLaunchKernelGGL(fill_pattern_kernel, dim3(4096), dim3(256), 0, e.s, dA.d(), tA.size(), 1.0);
+    hipLaunchKernelGGL(fill_pattern_kernel, dim3(4096), dim3(256), 0, e.s, dB.d(), tB.size(), 0.5);
+    size_t need = sizeof(double) * (size_t)(2 * (tA.size() + tB.size() + tC.size()) + 4096);
+    if (e.ops.arena.buf.bytes < need) e.ops.arena.buf.alloc(need);
+    e.ops.arena.reset();
+    e.ops.contract_p(static_cast<ecg::Prec>(prec), 1.0, tA, la, tB, lb, 0.0, tC, lc);  // warm-up
+    hipEvent_t a, b;
+    ECG_HIP(hipEventCreate(&a));
+    ECG_HIP(hipEventCreate(&b));
+    ECG_HIP(hipEventRecord(a, e.s));
+    for (int r = 0; r < reps; ++r) e.ops.contract_p(static_cast<ecg::Prec>(prec), 1.0, tA, la, tB, lb, 0.0, tC, lc);
+    ECG_HIP(hipEventRecord(b, e.s));
+    ECG_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    ECG_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+  } catch (const std::exception&) {
+    return -1.0;
+  }
+}
+
 // Lane-layout probe of the FP64 and FP32 16x16x4 MFMA builtins (64 lanes, in bounds).
 int ecgt_mfma_probe(const double* a, const double* b, double* out, const float* af, const float* bf, float* outf) {
   try {

@@ -51,6 +51,7 @@ _SIGS = {
     "ecg_lambda_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
     "ecg_one_rdm": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_profile": [_vp, ctypes.c_int],
     "ecg_stats": [_vp, _dp],
 }
 _RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p}
@@ -226,6 +227,10 @@ class Context:
         D1, dD1 = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
         self._chk(self._lib.ecg_one_rdm(self._h, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2), _ptr(D1), _ptr(dD1)))
         return D1, dD1
+
+    def profile(self, enable=True):
+        """Record HIP events around every MFMA GEMM launch (``ecg_profile``)."""
+        self._chk(self._lib.ecg_profile(self._h, int(enable)))
 
     def stats(self):
         s = np.zeros(5)

@@ -53,8 +53,7 @@ def make_system(nocc: int, nvirt: int, naux: int, seed: int):
         m = np.zeros((n, n))
         m[rows, cols] = g[Q, tri]
         B[:, :, Q] = m + np.triu(m, 1).T
-    Bo = B[:, :nocc, :]
     J = np.tensordot(B, np.einsum('iiQ->Q', B[:nocc, :nocc, :]), axes=([2], [0]))
-    K = np.einsum('piQ,iqQ->pq', Bo, B[:nocc, :, :])
+    K = np.tensordot(B[:, :nocc, :], B[:nocc, :, :], axes=([1, 2], [0, 2]))
     h = np.diag(eps) - (2.0 * J - K)
     return eps, np.asfortranarray(h), B, 0.0

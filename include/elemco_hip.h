@@ -106,7 +106,12 @@ int ecg_lambda_solve(ecg_ctx* ctx, int method, const ecg_solver_opts* opts, cons
 int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* U1, const double* U2,
                 double* D1, double* dD1);
 
-/* Timing of the last call: [residual_ms, ladder_ms, gemm_ms, gemm_launches, gemm_flops] */
+/* Instrumentation (no counterpart in the reference, whose timings come from
+ * print_time, src/tools/utils.jl:51-57): enable = 1 records HIP events around
+ * every MFMA GEMM launch on the library stream and zeroes the counters.
+ * stats = [last residual ms, last ladder ms, timed GEMM ms, timed GEMM launches,
+ *          issued GEMM flops] */
+int ecg_profile(ecg_ctx* ctx, int enable);
 int ecg_stats(ecg_ctx* ctx, double* stats);
 
 #ifdef __cplusplus
