@@ -57,6 +57,7 @@ struct Context {
   void residual(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d);
   // adds the (a <= b half of the) DF particle-particle ladder to R2r; T2x: amplitudes as [i,j,c,d]
   void ladder(Prec pr, const double* T2x, double* R2r);
+  void ladder_df(Prec pr, const double* Lf, const double* Rf, const double* T2x, double* R2r);
   // energies
   double hf_energy();
   void mp2_guess(double* T1d, double* T2d, double* emp2 /*[E,ESS,EOS]*/);

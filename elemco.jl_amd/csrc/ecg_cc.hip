@@ -283,6 +283,14 @@ void Context::dress(const double* T1d) {
 // inside the diagonal block are formed twice and weighted by 1/2.
 void Context::ladder(Prec pr, const double* T2x, double* R2r) {
   const Blocks& Rh = hermitian ? Ld : Rd;
+  ECG_HIP(hipEventRecord(ev[2], stream));
+  ladder_df(pr, Ld.vv.d(), Rh.vv.d(), T2x, R2r);
+  ECG_HIP(hipEventRecord(ev[3], stream));
+}
+
+// out[a,b,i,j] += sum_Q sum_cd Lf[a,c,Q] Rf[b,d,Q] X[c,d,i,j] for b >= (start of a's block),
+// diagonal blocks weighted 1/2 (see ladder); X is supplied as Xx[i,j,c,d].
+void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const double* T2x, double* R2r) {
   size_t mark = ops.arena.mark();
   size_t free_b = ops.arena.buf.bytes - ops.arena.mark();
   const long v = nv;
@@ -290,15 +298,14 @@ void Context::ladder(Prec pr, const double* T2x, double* R2r) {
   long ab = std::max<long>(1, std::min<long>(v, (long)(free_b * 0.8 / sizeof(double)) / per_a));
   double* V = ops.arena.take(ab * per_a);
   ops.gemm.stream = stream;
-  ECG_HIP(hipEventRecord(ev[2], stream));
   for (long a0 = 0; a0 < v; a0 += ab) {
     const long na = std::min<long>(ab, v - a0);
     const long nb = v - a0;
     // V[a',b',c,d] = sum_Q L^[a0+a',c,Q] R^[a0+b',d,Q]   (m = (a',c), n = (b',d), k = Q)
     GemmOp g;
     g.M = (int)(na * v); g.N = (int)(nb * v); g.K = nq;
-    g.A = Ld.vv.d() + a0; g.a_s0 = 1; g.a_d0 = (int)na; g.a_s1 = v; g.a_k = v * v;
-    g.B = Rh.vv.d() + a0; g.b_s0 = 1; g.b_d0 = (int)nb; g.b_s1 = v; g.b_k = v * v;
+    g.A = Lf + a0; g.a_s0 = 1; g.a_d0 = (int)na; g.a_s1 = v; g.a_k = v * v;
+    g.B = Rf + a0; g.b_s0 = 1; g.b_d0 = (int)nb; g.b_s1 = v; g.b_k = v * v;
     g.C = V; g.c_ms0 = 1; g.c_md0 = (int)na; g.c_ms1 = na * nb;
     g.c_ns0 = na; g.c_nd0 = (int)nb; g.c_ns1 = na * nb * v;
     g.alpha = 1.0; g.beta = 0.0;
@@ -319,7 +326,6 @@ void Context::ladder(Prec pr, const double* T2x, double* R2r) {
       ops.gemm.run_prec(h, pr);
     }
   }
-  ECG_HIP(hipEventRecord(ev[3], stream));
   ops.arena.release_to(mark);
 }
 
