@@ -350,7 +350,6 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   ops.axpby(nvvoo(), 2.0, T2, 0.0, Tt);
   ops.permute(tT2, p_ba, tTt, -1.0, 1.0);
   Tensor klcd = T4d(oovv.d(), o, o, v, v);
-  Tensor dfm = T2d(df.d(), n, n);
   // ---- singles (cc.jl:1357-1378 and the kext part :1413-1421)
   Tensor tR1 = T2d(R1d, v, o);
   {

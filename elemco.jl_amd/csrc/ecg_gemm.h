@@ -57,7 +57,4 @@ struct GemmEngine {
   void run_prec(const GemmOp& op, Prec p);
 };
 
-// FP32 / BF16-split products (ecg_gemm_lowp.hip).
-void gemm_lowprec(GemmEngine& eng, const GemmOp& op, Prec p);
-
 }  // namespace ecg

@@ -51,10 +51,34 @@ __device__ __forceinline__ void tile_coords(int wg, int nwg, int tiles_m, int& t
 
 constexpr int BM = 128, BN = 128, BKS = 16, LDP = BM + 16;
 
-template <bool AK, bool BK>
-__global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmParams p) {
-  __shared__ double As[2][BKS][LDP];
-  __shared__ double Bs[2][BKS][LDP];
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// Per-precision pieces: operand type in LDS, accumulator, the 16x16x4 MFMA and
+// the accumulator row layout (lane probe on gfx950: f64 slot r of lane l holds
+// row l/16 + 4r, f32 slot r holds row 4(l/16) + r; the column is l%16 for both).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4_t acc_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef f4_t acc_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+// T = double: FP64 MFMA.  T = float: operands rounded to FP32 when staged in
+// LDS, FP32 MFMA accumulation, FP64 epilogue (alpha, beta and C stay FP64).
+template <typename T, bool AK, bool BK>
+__global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[2][BKS][LDP];
+  __shared__ T Bs[2][BKS][LDP];
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -122,21 +146,21 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmParams p) {
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (AK) As[buf][t & 15][(t >> 4) + 16 * j] = ra[j];
-      else    As[buf][(t >> 7) + 2 * j][t & 127] = ra[j];
+      if (AK) As[buf][t & 15][(t >> 4) + 16 * j] = (T)ra[j];
+      else    As[buf][(t >> 7) + 2 * j][t & 127] = (T)ra[j];
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (BK) Bs[buf][t & 15][(t >> 4) + 16 * j] = rb[j];
-      else    Bs[buf][(t >> 7) + 2 * j][t & 127] = rb[j];
+      if (BK) Bs[buf][t & 15][(t >> 4) + 16 * j] = (T)rb[j];
+      else    Bs[buf][(t >> 7) + 2 * j][t & 127] = (T)rb[j];
     }
   };
 
-  d4_t acc[4][4];
+  acc_t acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int li = lane & 15, lk = lane >> 4;
   int cur = 0;
@@ -150,7 +174,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmParams p) {
     if (more) gload(k0 + BKS);
 #pragma unroll
     for (int kk = 0; kk < BKS; kk += 4) {
-      double a[4], b[4];
+      T a[4], b[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) a[i] = As[cur][kk + lk][wm * 64 + i * 16 + li];
 #pragma unroll
@@ -158,8 +182,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmParams p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
     }
     if (more) sstore(cur ^ 1);
     __syncthreads();
@@ -178,10 +201,10 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmParams p) {
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int m = bm + wm * 64 + i * 16 + lk + 4 * r;
+        int m = bm + wm * 64 + i * 16 + Mfma<T>::row(lk, r);
         if (m >= p.M) continue;
         long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
-        double v = p.alpha * acc[i][j][r];
+        double v = p.alpha * (double)acc[i][j][r];
         if (p.beta != 0.0) v += p.beta * C[cm + cn];
         C[cm + cn] = v;
       }
@@ -223,10 +246,7 @@ void GemmEngine::collect_timing() {
 void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
   if (op.M <= 0 || op.N <= 0 || op.batch <= 0) return;
   ECG_CHECK(op.K > 0, "gemm: K must be positive");
-  if (pr != Prec::F64) {
-    gemm_lowprec(*this, op, pr);
-    return;
-  }
+  ECG_CHECK(pr == Prec::F64 || pr == Prec::F32, "gemm: BF16 hi/lo products are not available in this build");
   flops += 2LL * op.M * op.N * (long long)op.K * op.batch;
   GemmParams p{};
   p.M = op.M; p.N = op.N; p.K = op.K;
@@ -271,10 +291,17 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     e1 = evpool[evused++];
     ECG_HIP(hipEventRecord(e0, stream));
   }
-  if (!ak && !bk) gemm_f64_kernel<false, false><<<grid, dim3(256), 0, stream>>>(p);
-  else if (ak && !bk) gemm_f64_kernel<true, false><<<grid, dim3(256), 0, stream>>>(p);
-  else if (!ak && bk) gemm_f64_kernel<false, true><<<grid, dim3(256), 0, stream>>>(p);
-  else gemm_f64_kernel<true, true><<<grid, dim3(256), 0, stream>>>(p);
+  if (pr == Prec::F64) {
+    if (!ak && !bk) gemm_kernel<double, false, false><<<grid, dim3(256), 0, stream>>>(p);
+    else if (ak && !bk) gemm_kernel<double, true, false><<<grid, dim3(256), 0, stream>>>(p);
+    else if (!ak && bk) gemm_kernel<double, false, true><<<grid, dim3(256), 0, stream>>>(p);
+    else gemm_kernel<double, true, true><<<grid, dim3(256), 0, stream>>>(p);
+  } else {
+    if (!ak && !bk) gemm_kernel<float, false, false><<<grid, dim3(256), 0, stream>>>(p);
+    else if (ak && !bk) gemm_kernel<float, true, false><<<grid, dim3(256), 0, stream>>>(p);
+    else if (!ak && bk) gemm_kernel<float, false, true><<<grid, dim3(256), 0, stream>>>(p);
+    else gemm_kernel<float, true, true><<<grid, dim3(256), 0, stream>>>(p);
+  }
   ECG_HIP(hipGetLastError());
   if (timed) ECG_HIP(hipEventRecord(e1, stream));
   if (ksplit > 1) {

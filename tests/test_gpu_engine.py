@@ -68,3 +68,17 @@ def test_gemm_large_split_k():
     ref = np.einsum("klcd,cdij->klij", A, B)
     got = gpu_contract(1.0, A, "klcd", B, "cdij", 0.0, np.zeros((40, 40, 40, 40)), "klij")
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-13
+
+
+@pytest.mark.parametrize("la,lb,lc", [("akic", "cbkj", "abij"), ("klcd", "cdij", "klij"), ("ad", "dbij", "abij")])
+def test_contract_fp32_products(la, lb, lc):
+    """FP32 MFMA products with FP32 accumulation: error bounded by K * 2^-23 relative."""
+    ext = SIZES["ragged"]
+    rng = np.random.default_rng(17)
+    A, B, C = _rand(la, ext, rng), _rand(lb, ext, rng), _rand(lc, ext, rng)
+    ref = np.einsum(f"{la},{lb}->{lc}", A, B) + C
+    got = gpu_contract(1.0, A, la, B, lb, 1.0, C, lc, prec=1)
+    K = np.prod([ext[c] for c in la if c not in lc])
+    # one rounding of each operand to FP32 plus FP32 accumulation, against the FP64 result
+    bound = 4.0 * np.sqrt(K) * 2.0**-23 * np.einsum(f"{la},{lb}->{lc}", np.abs(A), np.abs(B)).max()
+    assert float(np.max(np.abs(got - ref))) < bound
