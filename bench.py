@@ -1,13 +1,15 @@
 #!/usr/bin/env python3
-"""Benchmark of the closed-shell CCSD iteration on one MI355X.
+"""Benchmark of the closed-shell CCSD iteration and (T) on one MI355X.
 
 Workload (default): o=40, v=360, naux=1200, seed 2 -- the seeded synthetic
 density-fitted system the headline metric is quoted on.  A step is one
 device-resident CCSD iteration (residual, Hylleraas energy, amplitude update,
-DIIS), with integrals and amplitudes resident in HBM before the timed region.
+DIIS) with integrals and amplitudes resident in HBM before the timed region.
 
-Prints one JSON line (rank 0).  value = algorithmic TFLOP/s of the CCSD
-iteration (count in elemco_amd/flops.py); ms_per_step = time per iteration.
+Prints one JSON line.  value = algorithmic TFLOP/s of the FP64 CCSD iteration
+(count in elemco_amd/flops.py); ms_per_step = FP64 time per iteration.  The
+same line carries the mixed-precision (FP32-MFMA) iteration, (T) rates on a
+sample of triplets, the roofline of the dominant kernel and the CPU baseline.
 """
 import argparse
 import json
@@ -18,7 +20,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "elemco.jl_amd"))
 
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}   # dense MFMA TFLOP/s of MI355X
+PEAK = {"fp64": 78.6, "fp32": 157.3}   # dense MFMA TFLOP/s of MI355X
 CONFIGS = {"c2": (40, 360, 1200, 2), "c1": (20, 180, 600, 1), "c3": (30, 270, 900, 3), "c0": (5, 19, 72, 0)}
 
 
@@ -28,7 +30,6 @@ def log(*a):
 
 def cpu_baseline(sample=(16, 144, 480, 7)):
     """Time the CPU oracle (numpy/OpenBLAS restatement) on one CCSD residual of a bounded sample."""
-    import numpy as np
     sys.path.insert(0, ROOT)
     from oracle.ccsd import ECInfo, calc_fock_matrix, calc_MP2, calc_cc_resid
     from oracle.synthetic import make_df_system
@@ -39,18 +40,51 @@ def cpu_baseline(sample=(16, 144, 480, 7)):
     except Exception:
         cores = os.cpu_count() or 1
     o, v, q, seed = sample
-    s = make_df_system(o, v, q, seed)
-    EC = ECInfo.from_df(s)
+    EC = ECInfo.from_df(make_df_system(o, v, q, seed))
     calc_fock_matrix(EC)
     calc_MP2(EC)
-    T1, T2 = EC.files["T_vo"], EC.files["T_vvoo"]
     t0 = time.perf_counter()
-    calc_cc_resid(EC, T1, T2)
+    calc_cc_resid(EC, EC.files["T_vo"], EC.files["T_vvoo"])
     dt = time.perf_counter() - t0
-    return {"value": ccsd_iteration(o, v, q) / dt / 1e12, "unit": "TFLOP/s", "cores": int(cores), "kind": "port",
-            "sample": f"one CCSD residual of the numpy oracle at o={o} v={v} naux={q} seed={seed} "
-                      f"({dt:.1f} s; same algorithmic flop count as the GPU figure)",
-            "seconds": dt}
+    return {"value": round(ccsd_iteration(o, v, q) / dt / 1e12, 5), "unit": "TFLOP/s", "cores": int(cores),
+            "kind": "port",
+            "sample": f"one CCSD residual of the numpy/OpenBLAS oracle at o={o} v={v} naux={q} seed={seed} "
+                      f"({dt:.1f} s); rate uses the same algorithmic flop count as the GPU figure"}
+
+
+def time_ccsd(E, ctx, meth, prec, steps, warmup):
+    opts = E.default_opts()
+    ctx.state_init(opts=opts)
+    for k in range(warmup):
+        t = time.time()
+        info = ctx.state_step(meth, prec, opts)
+        log(f"  warmup {k}: {time.time() - t:.3f} s  E={info[0]:.10g}")
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        info = ctx.state_step(meth, prec, opts)
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.profile(False)
+    return dt / steps, st, info
+
+
+def time_triples(E, ctx, prec, nsample, o, v):
+    """(T) rate from triplets 8 .. 8+nsample-1: time(0, 8+n) - time(0, 8) excludes the integral set-up."""
+    import numpy as np
+    from elemco_amd.flops import triples, triplet_count
+    T1, T2, _ = ctx.mp2()
+    ctx.pert_triples_range(T1, T2, 0, 2, precision=prec)            # warm-up
+    t0 = time.perf_counter()
+    ctx.pert_triples_range(T1, T2, 0, 8, precision=prec)
+    t1 = time.perf_counter()
+    ctx.pert_triples_range(T1, T2, 0, 8 + nsample, precision=prec)
+    t2 = time.perf_counter()
+    dt = (t2 - t1) - (t1 - t0)
+    per = triples(o, v) / triplet_count(o)
+    return {"tflops": round(nsample * per / dt / 1e12, 3), "ms_per_triplet": round(dt / nsample * 1e3, 3),
+            "triplets_sampled": nsample, "triplets_total": triplet_count(o),
+            "full_T_estimate_s": round(dt / nsample * triplet_count(o), 1)}
 
 
 def main():
@@ -59,13 +93,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp64", choices=sorted(PEAK))
     ap.add_argument("--method", default="ccsd", choices=["ccsd", "dcsd"])
+    ap.add_argument("--triplets", type=int, default=48, help="(T) sample size; 0 skips (T)")
+    ap.add_argument("--no-mixed", action="store_true", help="time only the FP64 iteration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(sharding is described in DESIGN.md)", file=sys.stderr)
+              "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     import elemco_amd as E
@@ -76,51 +111,47 @@ def main():
     EC = E.ECInfo.synthetic(o, v, q, seed)
     log(f"setup o={o} v={v} naux={q}: {time.time() - t0:.1f} s")
     ctx = EC.ctx
-    prec = E.lib.PRECISIONS[args.precision]
     meth = E.ECG_DCSD if args.method == "dcsd" else E.ECG_CCSD
-    opts = E.default_opts()
-    ctx.state_init(opts=opts)
-    for k in range(args.warmup):
-        t = time.time()
-        info = ctx.state_step(meth, prec, opts)
-        log(f"warmup {k}: {time.time() - t:.3f} s  E={info[0]:.10f} NormR={info[5]:.3e}")
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        info = ctx.state_step(meth, prec, opts)
-        log(f"step {k}: E={info[0]:.10f} dE={info[4]:.3e} NormR={info[5]:.3e}")
-    dt = time.perf_counter() - t0
-    st = ctx.stats()
-    ctx.profile(False)
-    s_per_step = dt / args.steps
     flops = ccsd_iteration(o, v, q, dcsd=(args.method == "dcsd"))
-    tflops = flops / s_per_step / 1e12
-    gemm_ms_per_step = st["gemm_ms"] / args.steps
-    gemm_launch_avg_ms = st["gemm_ms"] / max(1, st["gemm_launches"])
-    launches_per_step = st["gemm_launches"] / args.steps
-    # dominant kernel: the MFMA GEMM; algorithmic flops of the iteration over the time spent in it
-    achieved = (flops / launches_per_step) / (gemm_launch_avg_ms * 1e-3) / 1e12
-    peak = PEAK[args.precision]
+
+    log("FP64 iterations")
+    s64, st, info = time_ccsd(E, ctx, meth, E.ECG_FP64, args.steps, args.warmup)
+    gemm_avg_ms = st["gemm_ms"] / max(1, st["gemm_launches"])
+    launches = st["gemm_launches"] / args.steps
+    achieved = (flops / launches) / (gemm_avg_ms * 1e-3) / 1e12
     line = {
         "metric": "s/CCSD-iter + (T) TFLOP/s, mixed vs FP64 GPU at |dE|<=1e-6 Eh, o40 v360",
-        "value": round(tflops, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(s_per_step * 1e3, 2), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[args.precision],
-        "data": "synthetic (seeded DF integrals, splitmix64)",
+        "value": round(flops / s64 / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(s64 * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded DF integrals, splitmix64; no checkpoints)",
         "config": {"workload": f"closed-shell {args.method.upper()} iteration, o={o} v={v} naux={q} seed={seed}",
-                   "nocc": o, "nvirt": v, "naux": q, "method": args.method, "precision": args.precision},
-        "roofline": {"bound": "mfma", "kernel": "gemm_f64_kernel" if args.precision == "fp64" else "gemm_lowp",
-                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "launches_per_step": launches_per_step, "avg_launch_ms": round(gemm_launch_avg_ms, 4),
-                     "gemm_ms_per_step": round(gemm_ms_per_step, 2),
-                     "issued_gemm_tflop_per_step": round(st["gemm_flops"] / args.steps / 1e12, 3)},
-        "energy": {"E_hylleraas": info[0], "NormR": info[5]},
+                   "nocc": o, "nvirt": v, "naux": q, "method": args.method, "precision": "fp64",
+                   "algorithmic_tflop_per_step": round(flops / 1e12, 3)},
+        "roofline": {"bound": "mfma", "kernel": "ecg::gemm_kernel<double,...> (all FP64 MFMA GEMM launches)",
+                     "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                     "launches_per_step": launches, "avg_launch_ms": round(gemm_avg_ms, 4),
+                     "gemm_ms_per_step": round(st["gemm_ms"] / args.steps, 2),
+                     "issued_tflop_per_step": round(st["gemm_flops"] / args.steps / 1e12, 3)},
     }
+    if not args.no_mixed:
+        log("FP32-MFMA iterations")
+        s32, st32, _ = time_ccsd(E, ctx, meth, E.ECG_FP32, args.steps, args.warmup)
+        line["mixed"] = {"precision": "fp32-mfma products, fp64 amplitudes/denominators/DIIS",
+                         "ms_per_step": round(s32 * 1e3, 2), "tflops": round(flops / s32 / 1e12, 3),
+                         "speedup_vs_fp64": round(s64 / s32, 3),
+                         "gemm_frac_of_fp32_peak": round((flops / (st32["gemm_launches"] / args.steps))
+                                                         / (st32["gemm_ms"] / max(1, st32["gemm_launches"]) * 1e-3)
+                                                         / 1e12 / PEAK["fp32"], 4)}
+    if args.triplets > 0:
+        log("(T) samples")
+        line["triples"] = {"fp64": time_triples(E, ctx, E.ECG_FP64, args.triplets, o, v)}
+        if not args.no_mixed:
+            line["triples"]["fp32"] = time_triples(E, ctx, E.ECG_FP32, args.triplets, o, v)
     if not args.no_cpu and args.config != "c0":
+        log("CPU baseline")
         try:
-            cb = cpu_baseline()
-            line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            line["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the baseline is reported, never required
             line["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(line), flush=True)

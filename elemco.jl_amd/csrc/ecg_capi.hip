@@ -197,7 +197,20 @@ int ecg_pert_triples(ecg_ctx* ctx, int lambda, int precision, const double* T1, 
     ECG_CHECK(T1 && T2, "NULL amplitudes");
     ECG_CHECK(!lambda || (U1 && U2), "Lambda-(T) needs U1 and U2");
     double e[2];
-    ecg::pert_triples(c, lambda != 0, to_prec(precision), T1, T2, U1, U2, e);
+    ecg::pert_triples(c, lambda != 0, to_prec(precision), T1, T2, U1, U2, 0, -1, e);
+    if (ET) *ET = e[0];
+    if (ETb) *ETb = e[1];
+  });
+}
+
+int ecg_pert_triples_range(ecg_ctx* ctx, int lambda, int precision, const double* T1, const double* T2,
+                           const double* U1, const double* U2, long first, long count, double* ET, double* ETb) {
+  return guarded(ctx, [&](ecg::Context& c) {
+    ECG_CHECK(T1 && T2, "NULL amplitudes");
+    ECG_CHECK(!lambda || (U1 && U2), "Lambda-(T) needs U1 and U2");
+    ECG_CHECK(first >= 0 && count >= 0, "triplet range must be non-negative");
+    double e[2];
+    ecg::pert_triples(c, lambda != 0, to_prec(precision), T1, T2, U1, U2, first, count, e);
     if (ET) *ET = e[0];
     if (ETb) *ETb = e[1];
   });

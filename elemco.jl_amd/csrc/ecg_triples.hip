@@ -234,7 +234,7 @@ void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const do
 }  // namespace
 
 void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const double* T2h, const double* U1h,
-                  const double* U2h, double* e) {
+                  const double* U2h, long first, long count, double* e) {
   const int v = c.nv, o = c.no;
   const long n1 = c.nvo(), n2 = c.nvvoo();
   const long vvv = (long)v * v * v;
@@ -299,10 +299,14 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   ca.Y = lambda ? dU2.d() : dT2.d();
   ca.part = part.d();
   // loop order of cc_triples.jl:61-71
+  long tidx = -1;
+  const long last = count < 0 ? (1L << 62) : first + count;
   for (int k = 0; k < o; ++k)
     for (int j = 0; j <= k; ++j)
       for (int i = 0; i <= j; ++i) {
         if (i == j && j == k) continue;
+        ++tidx;
+        if (tidx < first || tidx >= last) continue;
         double w = (i < j && j < k) ? 2.0 : 1.0;
         build_K(ops.gemm, pr, v, o, i, j, k, dT2.d(), vvvo.d(), ovoo.d(), K.d());
         if (lambda) build_K(ops.gemm, pr, v, o, i, j, k, dU2.d(), WL, HL, KU.d());

@@ -72,8 +72,9 @@ class ECInfo:
         return cls(fd.nelec // 2, fd.int1, BL, BR, fd.int0, **kw)
 
     @classmethod
-    def synthetic(cls, nocc, nvirt, naux, seed, **kw):
-        eps, h, B, e0 = make_system(nocc, nvirt, naux, seed)
+    def synthetic(cls, nocc, nvirt, naux, seed, scale=1.0, **kw):
+        """Seeded synthetic DF system; ``scale`` multiplies the 0.05 factor width."""
+        eps, h, B, e0 = make_system(nocc, nvirt, naux, seed, scale)
         return cls(nocc, h, B, None, e0, **kw)
 
     def opts(self):

@@ -48,6 +48,8 @@ _SIGS = {
     "ecg_cc_state_get": [_vp, _dp, _dp],
     "ecg_cc_solve": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _ip, _ip],
     "ecg_pert_triples": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_pert_triples_range": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_long,
+                               _dp, _dp],
     "ecg_lambda_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
     "ecg_one_rdm": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
@@ -203,6 +205,18 @@ class Context:
         et, etb = ctypes.c_double(), ctypes.c_double()
         self._chk(self._lib.ecg_pert_triples(self._h, int(lam), precision, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2),
                                              ctypes.byref(et), ctypes.byref(etb)))
+        return et.value, etb.value
+
+    def pert_triples_range(self, T1, T2, first, count, U1=None, U2=None, precision=ECG_FP64):
+        """(T) over triplets first..first+count-1 of the reference loop order (partial sums)."""
+        lam = U1 is not None
+        T1, T2 = self._t1(T1), self._t2(T2)
+        if lam:
+            U1, U2 = self._t1(U1), self._t2(U2)
+        et, etb = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._lib.ecg_pert_triples_range(self._h, int(lam), precision, _ptr(T1), _ptr(T2), _ptr(U1),
+                                                   _ptr(U2), int(first), int(count), ctypes.byref(et),
+                                                   ctypes.byref(etb)))
         return et.value, etb.value
 
     def lambda_residual(self, T1, T2, U1, U2, method=ECG_CCSD, precision=ECG_FP64):

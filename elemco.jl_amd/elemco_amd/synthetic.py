@@ -39,13 +39,23 @@ def _normal(seed, stream, count):
     return np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
 
 
-def make_system(nocc: int, nvirt: int, naux: int, seed: int):
-    """Return (eps, h, B, e0) with B of shape (n, n, naux) in column-major order."""
+def conditioned_scale(nocc: int, nvirt: int, naux: int) -> float:
+    """Width factor keeping o^2 v^2 Q sigma^4 at its o=5, v=19, Q=72 value (E_MP2 near -2 Eh).
+
+    The systems with the plain width 0.05 have E_MP2 = -2.1e4 Eh at o=20, v=180,
+    Q=600 and a divergent CCSD; they serve as timing workloads, the conditioned
+    ones for converged energies.
+    """
+    return (5.0**2 * 19.0**2 * 72.0 / (float(nocc) ** 2 * float(nvirt) ** 2 * float(naux))) ** 0.25
+
+
+def make_system(nocc: int, nvirt: int, naux: int, seed: int, scale: float = 1.0):
+    """Return (eps, h, B, e0) with B of shape (n, n, naux) in column-major order (width 0.05*scale)."""
     n = nocc + nvirt
     eps = np.concatenate([np.sort(-1.5 + 1.2 * _uniform(seed, 1, nocc)),
                           np.sort(0.2 + 2.8 * _uniform(seed, 2, nvirt))])
     ntri = n * (n + 1) // 2
-    g = 0.05 * _normal(seed, 3, ntri * naux).reshape(naux, ntri)
+    g = (0.05 * scale) * _normal(seed, 3, ntri * naux).reshape(naux, ntri)
     rows, cols = np.triu_indices(n)
     tri = rows + cols * (cols + 1) // 2
     B = np.empty((n, n, naux), order="F")

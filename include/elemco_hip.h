@@ -91,6 +91,13 @@ int ecg_cc_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts*
  * needs the contravariant multipliers U1, U2.  Returns E(T) and E[T]. */
 int ecg_pert_triples(ecg_ctx* ctx, int lambda, int precision, const double* T1, const double* T2,
                      const double* U1, const double* U2, double* ET, double* ETb);
+/* The same restricted to triplets first .. first+count-1 of the reference's loop
+ * order (cc_triples.jl:61-71: k, j<=k, i<=j, i=j=k skipped).  E(T) and E[T] are
+ * sums over triplets, so the results of disjoint ranges add up: this is the unit
+ * in which (T) is distributed over devices or processes. */
+int ecg_pert_triples_range(ecg_ctx* ctx, int lambda, int precision, const double* T1, const double* T2,
+                           const double* U1, const double* U2, long first, long count, double* ET,
+                           double* ETb);
 
 /* Lambda residual R = U x Jacobian: calc_ccsd_vector_times_Jacobian(EC, U1, U2; dc)
  * (src/cc/cc_lagrange.jl:9-212) at amplitudes T1, T2. */

@@ -75,15 +75,25 @@ class DFSystem:
         return np.diag(self.eps)
 
 
-def make_df_system(nocc: int, nvirt: int, naux: int, seed: int) -> DFSystem:
-    """Build the synthetic system of the given sizes and seed."""
+def conditioned_scale(nocc: int, nvirt: int, naux: int) -> float:
+    """Factor on the 0.05 width that keeps o^2 v^2 Q sigma^4 at its o=5, v=19, Q=72 value.
+
+    With sigma = 0.05 the MP2 energy grows like o^2 v^2 Q sigma^4 (-754 Eh at
+    o=10, v=90, Q=300; -2.1e4 Eh at o=20, v=180, Q=600) and CCSD diverges; the
+    conditioned systems keep it near the o=5 value (about -2 Eh).
+    """
+    return (5.0**2 * 19.0**2 * 72.0 / (float(nocc) ** 2 * float(nvirt) ** 2 * float(naux))) ** 0.25
+
+
+def make_df_system(nocc: int, nvirt: int, naux: int, seed: int, scale: float = 1.0) -> DFSystem:
+    """Build the synthetic system of the given sizes and seed (factor width 0.05*scale)."""
     n = nocc + nvirt
     eo = np.sort(-1.5 + 1.2 * uniform01(seed, STREAM_EOCC, nocc))
     ev = np.sort(0.2 + 2.8 * uniform01(seed, STREAM_EVIRT, nvirt))
     eps = np.concatenate([eo, ev])
     ntri = n * (n + 1) // 2
     # draw order: for Q, for q (column), for p <= q  -> flat upper-triangle index p + q(q+1)/2
-    g = 0.05 * normal(seed, STREAM_B, ntri * naux).reshape(naux, ntri)
+    g = (0.05 * scale) * normal(seed, STREAM_B, ntri * naux).reshape(naux, ntri)
     iu = np.triu_indices(n)            # row p <= col q
     tri = iu[0] + iu[1] * (iu[1] + 1) // 2
     B = np.empty((n, n, naux))
