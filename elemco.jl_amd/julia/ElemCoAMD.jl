@@ -1,0 +1,186 @@
+"""
+    ElemCoAMD
+
+Julia binding of `libelemco_hip.so`, the MI355X (gfx950) library of the
+closed-shell coupled-cluster path of ElemCo.jl.  Every wrapper is a `ccall` of
+one entry point of `include/elemco_hip.h`; arrays are passed as Julia
+column-major `Float64` arrays (the layout the library expects), and a negative
+status is raised with `error(...)`, the reference's convention (e.g.
+`src/cc/cc.jl:1364`).
+
+The drop-in methods at the end keep the reference's signatures:
+
+- `calc_cc_resid(ctx, T1, T2; dc)`            ↔ `src/cc/cc.jl:1343-1489`
+- `calc_ccsd_vector_times_Jacobian(ctx, …)`  ↔ `src/cc/cc_lagrange.jl:9-212`
+- `calc_1RDM(ctx, U1, U2, T1, T2)`          ↔ `src/cc/cc_lagrange.jl:793-837`
+- `calc_pertT(ctx, T1, T2; U1, U2)`          ↔ `src/cc/cc_triples.jl:39-288`
+- `calc_cc(ctx, method)`                     ↔ `src/cc/cc.jl:2451-2516`
+- `calc_lm_cc(ctx, T1, T2, method)`          ↔ `src/cc/cc_lagrange.jl:1120-1223`
+
+`ctx` stands for the reference's `EC::ECInfo` on this path: it owns the
+integrals resident on one GPU.  Build it from MO density-fitting factors
+`B[p,q,Q]` (the layout of `mmL`, `src/integrals/dftools.jl:102`) and the
+one-electron integrals.
+"""
+module ElemCoAMD
+
+export ECGContext, calc_cc_resid, calc_ccsd_vector_times_Jacobian, calc_1RDM, calc_pertT, calc_cc, calc_lm_cc,
+       SolverOpts, default_opts
+
+const LIB = get(ENV, "ELEMCO_HIP_LIB", joinpath(@__DIR__, "..", "libelemco_hip.so"))
+
+const ECG_CCSD = Cint(0)
+const ECG_DCSD = Cint(1)
+const ECG_FP64 = Cint(0)
+const ECG_FP32 = Cint(1)
+
+"Mirror of `ecg_solver_opts` (defaults of `src/infos/options.jl:128-221,272-282`)."
+mutable struct SolverOpts
+  thr::Cdouble
+  conven::Cdouble
+  shifts::Cdouble
+  shiftp::Cdouble
+  resthr::Cdouble
+  maxit::Cint
+  maxdiis::Cint
+  refine_every::Cint
+end
+
+function default_opts(; kw...)
+  o = SolverOpts(0, 0, 0, 0, 0, 0, 0, 0)
+  ccall((:ecg_default_opts, LIB), Cvoid, (Ref{SolverOpts},), o)
+  for (k, v) in kw
+    setfield!(o, k, convert(fieldtype(SolverOpts, k), v))
+  end
+  return o
+end
+
+mutable struct ECGContext
+  handle::Ptr{Cvoid}
+  nocc::Int
+  nvirt::Int
+  naux::Int
+end
+
+function _check(ctx, st)
+  if st != 0
+    msg = unsafe_string(ccall((:ecg_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx === nothing ? C_NULL : ctx.handle))
+    error(msg)
+  end
+  return nothing
+end
+
+"""
+    ECGContext(nocc, h, BL; BR=nothing, e0=0.0, device=0)
+
+Upload the one-electron integrals `h` (n×n) and DF factors `BL` (n×n×naux;
+`BR` for non-Hermitian integrals) to HIP device `device` (`ecg_create`).
+"""
+function ECGContext(nocc::Integer, h::Matrix{Float64}, BL::Array{Float64,3}; BR=nothing, e0=0.0, device=0)
+  n = size(h, 1)
+  naux = size(BL, 3)
+  size(BL, 1) == n && size(BL, 2) == n || error("BL must be n×n×naux")
+  ref = Ref{Ptr{Cvoid}}(C_NULL)
+  brp = BR === nothing ? Ptr{Cdouble}(C_NULL) : pointer(BR)
+  st = GC.@preserve BR ccall((:ecg_create, LIB), Cint,
+                             (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble, Cint,
+                              Csize_t),
+                             ref, nocc, n - nocc, naux, h, BL, brp, e0, device, 0)
+  _check(nothing, st)
+  ctx = ECGContext(ref[], nocc, n - nocc, naux)
+  finalizer(c -> (c.handle != C_NULL && ccall((:ecg_destroy, LIB), Cint, (Ptr{Cvoid},), c.handle); c.handle = C_NULL), ctx)
+  return ctx
+end
+
+_z1(ctx) = zeros(ctx.nvirt, ctx.nocc)
+_z2(ctx) = zeros(ctx.nvirt, ctx.nvirt, ctx.nocc, ctx.nocc)
+
+"`calc_cc_resid(EC, T1, T2; dc)` (src/cc/cc.jl:1343-1489): returns freshly allocated `(R1, R2)`."
+function calc_cc_resid(ctx::ECGContext, T1::Matrix{Float64}, T2::Array{Float64,4}; dc=false, precision=ECG_FP64)
+  R1, R2 = _z1(ctx), _z2(ctx)
+  st = ccall((:ecg_cc_residual, LIB), Cint,
+             (Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+             ctx.handle, dc ? ECG_DCSD : ECG_CCSD, precision, T1, T2, R1, R2)
+  _check(ctx, st)
+  return R1, R2
+end
+
+"`calc_ccsd_vector_times_Jacobian(EC, U1, U2; dc)` (src/cc/cc_lagrange.jl:9-212) at amplitudes T1, T2."
+function calc_ccsd_vector_times_Jacobian(ctx::ECGContext, U1, U2, T1, T2; dc=false, precision=ECG_FP64)
+  R1, R2 = _z1(ctx), _z2(ctx)
+  st = ccall((:ecg_lambda_residual, LIB), Cint,
+             (Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+              Ptr{Cdouble}),
+             ctx.handle, dc ? ECG_DCSD : ECG_CCSD, precision, T1, T2, U1, U2, R1, R2)
+  _check(ctx, st)
+  return R1, R2
+end
+
+"`calc_1RDM(EC, U1, U2, T1, T2)` (src/cc/cc_lagrange.jl:793-837): returns `(D1, dD1)`."
+function calc_1RDM(ctx::ECGContext, U1, U2, T1, T2)
+  n = ctx.nocc + ctx.nvirt
+  D1, dD1 = zeros(n, n), zeros(n, n)
+  st = ccall((:ecg_one_rdm, LIB), Cint,
+             (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+             ctx.handle, T1, T2, U1, U2, D1, dD1)
+  _check(ctx, st)
+  return D1, dD1
+end
+
+"""
+`calc_pertT` (src/cc/cc_triples.jl:39-288): `Dict("ET3" => E(T), "ET3b" => E[T])`;
+Λ-(T) when `U1`, `U2` are given.  `first`/`count` restrict the evaluation to a
+range of triplets of the reference's loop order (partial sums add up).
+"""
+function calc_pertT(ctx::ECGContext, T1, T2; U1=nothing, U2=nothing, precision=ECG_FP64, first=0, count=-1)
+  lam = U1 !== nothing
+  et, etb = Ref{Cdouble}(0), Ref{Cdouble}(0)
+  u1 = lam ? U1 : Float64[]
+  u2 = lam ? U2 : Float64[]
+  st = if count < 0
+    ccall((:ecg_pert_triples, LIB), Cint,
+          (Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Cdouble}, Ref{Cdouble}),
+          ctx.handle, lam, precision, T1, T2, u1, u2, et, etb)
+  else
+    ccall((:ecg_pert_triples_range, LIB), Cint,
+          (Ptr{Cvoid}, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Clong, Clong, Ref{Cdouble},
+           Ref{Cdouble}),
+          ctx.handle, lam, precision, T1, T2, u1, u2, first, count, et, etb)
+  end
+  _check(ctx, st)
+  return Dict("ET3" => et[], "ET3b" => etb[])
+end
+
+"""
+`calc_cc(EC, method)` (src/cc/cc.jl:2451-2516) for "ccsd" / "dcsd": iterate on the
+device from the MP2 guess; returns `(energies, T1, T2)` with
+`energies = Dict("E" => E_H, "ESS", "EOS", "Eproj", "pESS", "pEOS")`.
+"""
+function calc_cc(ctx::ECGContext, method::AbstractString; opts=default_opts(), precision=ECG_FP64)
+  m = lowercase(method) == "dcsd" ? ECG_DCSD : ECG_CCSD
+  T1, T2 = _z1(ctx), _z2(ctx)
+  e = zeros(6)
+  it, conv = Ref{Cint}(0), Ref{Cint}(0)
+  st = ccall((:ecg_cc_solve, LIB), Cint,
+             (Ptr{Cvoid}, Cint, Cint, Ref{SolverOpts}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Cint}, Ref{Cint}),
+             ctx.handle, m, precision, opts, T1, T2, e, it, conv)
+  _check(ctx, st)
+  conv[] == 0 && @warn "CC iterations did not converge in $(it[]) iterations"
+  return Dict("E" => e[1], "ESS" => e[2], "EOS" => e[3], "Eproj" => e[4], "pESS" => e[5], "pEOS" => e[6]), T1, T2
+end
+
+"`calc_lm_cc(EC, method)` (src/cc/cc_lagrange.jl:1120-1223): Λ multipliers `(U1, U2)` at amplitudes T1, T2."
+function calc_lm_cc(ctx::ECGContext, T1, T2, method::AbstractString; opts=default_opts())
+  m = lowercase(method) == "dcsd" ? ECG_DCSD : ECG_CCSD
+  U1, U2 = _z1(ctx), _z2(ctx)
+  it, conv = Ref{Cint}(0), Ref{Cint}(0)
+  st = ccall((:ecg_lambda_solve, LIB), Cint,
+             (Ptr{Cvoid}, Cint, Ref{SolverOpts}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Cint},
+              Ref{Cint}),
+             ctx.handle, m, opts, T1, T2, U1, U2, it, conv)
+  _check(ctx, st)
+  conv[] == 0 && @warn "Lambda iterations did not converge in $(it[]) iterations"
+  return U1, U2
+end
+
+end # module

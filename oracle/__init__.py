@@ -18,10 +18,13 @@ the ElemCo.jl v0.14.0 source tree):
 
 and generates the seeded synthetic density-fitted systems (synthetic.py).
 
-Parity pins (each function's tests live in ``tests/test_oracle_*.py``):
-  - reference-held energies of ``test/h2o.jl`` and ``test/h2o_st1.jl`` (1e-6 Eh);
-  - two-electron systems, where CCSD and DCSD equal full CI (closed form);
-  - a spin-orbital textbook CCSD/(T) implementation on tiny systems;
+Parity pins (tests in ``tests/test_oracle.py``):
+  - reference-held energies of ``test/h2o.jl`` and ``test/h2o_st1.jl`` (1e-6 Eh):
+    HF, MP2, CCSD, CCSD(T), DCSD, Lambda-CCSD(T), Hermitian and ST files;
+  - two-electron systems, where CCSD and DCSD equal full CI (closed form) and
+    E(T) vanishes (every triplet has i = j = k);
   - brute-force [T] from explicit T3 (the reference's ``cc_tests.jl:54-75``);
-  - finite differences of the CC Lagrangian for the Lambda residual and 1-RDM.
+  - finite differences of the CC Lagrangian for the Lambda residual and of the
+    energy for the 1-RDM;
+  - splitmix64 published vectors for the synthetic generator.
 """
