@@ -1,8 +1,12 @@
 // extern "C" boundary of the library (declarations and reference citations in include/elemco_hip.h).
 #include "../../include/elemco_hip.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
+#include <thread>
+#include <vector>
 
 #include "ecg_cc.h"
 #include "ecg_lambda.h"
@@ -191,6 +195,40 @@ int ecg_cc_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts*
   });
 }
 
+int ecg_cc_solve_batch(int nsys, ecg_ctx* const* ctxs, int method, int precision, const ecg_solver_opts* opts,
+                       int max_threads, double* energies, int* iterations, int* converged, int* status) {
+  if (nsys < 0 || (nsys > 0 && (!ctxs || !energies || !iterations || !converged))) return -1;
+  for (int k = 0; k < nsys; ++k)
+    if (!ctxs[k]) return -1;
+  std::vector<int> st(nsys, 0);
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    for (int k = next.fetch_add(1); k < nsys; k = next.fetch_add(1)) {
+      st[k] = guarded(ctxs[k], [&](ecg::Context& c) {
+        check_method(method);
+        ecg::SolverOpts o = to_opts(opts);
+        c.state_init(nullptr, nullptr, o);
+        int conv = 0;
+        iterations[k] = c.solve(method, to_prec(precision), o, energies + 6 * k, &conv, nullptr);
+        converged[k] = conv;
+        ECG_HIP(hipStreamSynchronize(c.stream));
+      });
+    }
+  };
+  int nt = max_threads > 0 ? max_threads : 16;
+  nt = std::max(1, std::min(nt, nsys));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  int first = 0;
+  for (int k = 0; k < nsys; ++k) {
+    if (status) status[k] = st[k];
+    if (st[k] != 0 && first == 0) first = st[k];
+  }
+  return first;
+}
+
 int ecg_pert_triples(ecg_ctx* ctx, int lambda, int precision, const double* T1, const double* T2, const double* U1,
                      const double* U2, double* ET, double* ETb) {
   return guarded(ctx, [&](ecg::Context& c) {
@@ -266,6 +304,7 @@ int ecg_stats(ecg_ctx* ctx, double* stats) {
     stats[2] = c.ops.gemm.timed_ms;
     stats[3] = (double)c.ops.gemm.timed_launches;
     stats[4] = (double)c.ops.gemm.flops;
+    stats[5] = (double)c.fp64_iterations;
   });
 }
 

@@ -80,7 +80,12 @@ struct Context {
   int iter = 0;
   void state_init(const double* T1h, const double* T2h, const SolverOpts& o);
   // info = [E_H, ESS, EOS, E_proj, dE, NormR, NormT, converged]
-  void state_step(int method, Prec pr, const SolverOpts& o, double* info);
+  // Low precision: the residual is R_low(T) + delta, delta = R_64(T_ref) - R_low(T_ref)
+  // from the last refresh (none after state_init); refresh = true evaluates both at
+  // the current T, stores the new delta and uses the exact FP64 residual.
+  void state_step(int method, Prec pr, const SolverOpts& o, double* info, bool refresh = false);
+  DevBuf D1, D2;             // FP64 residual correction (defect) of the mixed-precision solver
+  bool have_delta = false;
   int solve(int method, Prec pr, const SolverOpts& o, double* energies, int* converged, FILE* log);
   // <ab|ij> as [a,b,i,j] (equals vvoo for Hermitian integrals)
   DevBuf abij_buf;
@@ -88,6 +93,7 @@ struct Context {
   // timing of the last residual call (HIP events on the library stream)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_residual_ms = 0.0, last_ladder_ms = 0.0;
+  int fp64_iterations = 0;   // FP64 iterations of the last solve (mixed precision)
   ~Context();
 };
 

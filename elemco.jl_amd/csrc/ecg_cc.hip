@@ -689,6 +689,7 @@ Context::~Context() {
 
 void Context::state_init(const double* T1h, const double* T2h, const SolverOpts& o) {
   const long n1 = nvo(), n2 = nvvoo();
+  have_delta = false;
   T1.ensure(sizeof(double) * n1);
   T2.ensure(sizeof(double) * n2);
   R1.ensure(sizeof(double) * n1);
@@ -706,12 +707,25 @@ void Context::state_init(const double* T1h, const double* T2h, const SolverOpts&
 }
 
 // One pass of the loop body of cc_iterations! (cc.jl:2546-2629), closed-shell branch.
-void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info) {
+void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info, bool refresh) {
   const long n1 = nvo();
   iter++;
-  Prec p = pr;
-  if (pr != Prec::F64 && o.refine_every > 0 && (iter % o.refine_every) == 0) p = Prec::F64;
-  residual(method, p, T1.d(), T2.d(), R1.d(), R2.d());
+  residual(method, pr, T1.d(), T2.d(), R1.d(), R2.d());
+  if (pr != Prec::F64) {
+    if (refresh) {
+      // delta = R_64 - R_low at this T; continue with the exact residual R_low + delta = R_64
+      D1.ensure(sizeof(double) * n1);
+      D2.ensure(sizeof(double) * nvvoo());
+      residual(method, Prec::F64, T1.d(), T2.d(), D1.d(), D2.d());
+      ops.axpby(n1, -1.0, R1.d(), 1.0, D1.d());
+      ops.axpby(nvvoo(), -1.0, R2.d(), 1.0, D2.d());
+      have_delta = true;
+    }
+    if (have_delta) {
+      ops.axpby(n1, 1.0, D1.d(), 1.0, R1.d());
+      ops.axpby(nvvoo(), 1.0, D2.d(), 1.0, R2.d());
+    }
+  }
   // calc_doubles_norm (cctools.jl:477-480): sum (2T - T_ba) T
   double NormT2 = 2.0 * doubles_dot(T2.d(), T2.d(), 0.5);
   double NormR2 = 2.0 * doubles_dot(R2.d(), R2.d(), 0.5);
@@ -735,26 +749,47 @@ void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info)
 
 // cc_iterations! closed-shell (cc.jl:2518-2648) from the amplitudes already in T1/T2.
 // energies: [E_H, ESS, EOS, E_proj, dE, NormR]; returns the number of iterations.
+// Mixed precision (pr != F64), defect correction: iterations with the residual
+// R_low + delta; delta is refreshed with an FP64 residual (an exact iteration)
+// every refine_every-th iteration (when > 0), whenever the corrected residual
+// norm drops below 100*thr, and whenever it stalls (less than 20% below the
+// smallest norm since the last refresh: the low-precision rounding floor).
+// Convergence is declared only on an exact (FP64) iteration, so the fixed
+// point is the FP64 solution.
 int Context::solve(int method, Prec pr, const SolverOpts& o, double* energies, int* converged, FILE* log) {
   diis_reset(o.maxdiis, o.resthr);
   iter = 0;
+  have_delta = false;
   double info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int conv = 0, it = 0;
+  const bool mixed = (pr != Prec::F64);
+  bool refresh_next = false;
+  double best = HUGE_VAL;
+  fp64_iterations = 0;
   for (it = 1; it <= o.maxit; ++it) {
-    state_step(method, pr, o, info);
+    const bool refresh =
+        mixed && (refresh_next || (o.refine_every > 0 && (it % o.refine_every) == 0));
+    const bool exact = !mixed || refresh;
+    state_step(method, pr, o, info, refresh);
+    if (exact) fp64_iterations++;
     if (log) {
-      std::fprintf(log, "%4d %12.8f %16.10f %12.3e %12.3e\n", it, info[6], info[0], info[4], info[5]);
+      std::fprintf(log, "%4d %12.8f %16.10f %12.3e %12.3e %s\n", it, info[6], info[0], info[4], info[5],
+                   exact ? "fp64" : "low");
       std::fflush(log);
     }
-    if (info[7] != 0.0) {
+    if (info[7] != 0.0 && exact) {
       conv = 1;
       break;
     }
+    if (mixed) {
+      if (refresh) best = HUGE_VAL;
+      const bool stalled = info[5] > 0.8 * best;
+      best = std::min(best, info[5]);
+      refresh_next = info[5] < 100.0 * o.thr || stalled;
+    }
   }
   if (it > o.maxit) it = o.maxit;
-  for (int k = 0; k < 6; ++k) energies[k] = info[k < 4 ? k : k];
-  energies[4] = info[4];
-  energies[5] = info[5];
+  for (int k = 0; k < 6; ++k) energies[k] = info[k];
   *converged = conv;
   return it;
 }

@@ -8,10 +8,24 @@
 // so permuted outputs (the (T) intermediates, the DF ladder blocks) are written
 // in place without a separate transpose pass.  An optional batch index z adds
 // z*a_z, z*b_z, z*c_z to the three base pointers.
+//
+// Segmented K: with nseg > 0 the product is a sum over up to MAXSEG segments,
+//     C = alpha * sum_s sum_{k < seg[s].K} A_s(m,k) B_s(k,n) + beta * C,
+// where segment s has its own base pointers and K strides (seg[s].A, a_k, B,
+// b_k) and shares the m / n addressing of the op.  One launch then accumulates
+// several products into the same output tile (the (T) intermediates).
 #pragma once
 #include "ecg_common.h"
 
 namespace ecg {
+
+constexpr int MAXSEG = 4;
+
+struct GemmSeg {
+  const double* A = nullptr; long a_k = 0;
+  const double* B = nullptr; long b_k = 0;
+  int K = 0;
+};
 
 struct GemmOp {
   int M = 0, N = 0, K = 0, batch = 1;
@@ -20,6 +34,8 @@ struct GemmOp {
   double* C = nullptr; long c_ms0 = 1, c_ms1 = 0, c_ns0 = 0, c_ns1 = 0, c_z = 0;
   int c_md0 = 1 << 30, c_nd0 = 1 << 30;
   double alpha = 1.0, beta = 0.0;
+  int nseg = 0;            // 0: single product (A, a_k, B, b_k, K); else seg[0..nseg)
+  GemmSeg seg[MAXSEG];
 
   // Plain column-major helpers: A is M x K (lda), or K x M when transposed.
   GemmOp& a_colmajor(const double* p, long lda, bool trans) {
@@ -55,6 +71,13 @@ struct GemmEngine {
   void collect_timing();   // synchronises and accumulates recorded launches
   void run(const GemmOp& op);
   void run_prec(const GemmOp& op, Prec p);
+  // BF16 hi/lo products (ecg_gemm_bf16.hip): operands packed into zero-padded,
+  // K-contiguous BF16 hi and lo planes in `pk`, 256x128 MFMA tiles
+  DevBuf pk;
+  void run_bf16x3(const GemmOp& op);
+  void reduce_slabs(const GemmOp& op, int ksplit);   // split-K partial slabs in ws -> C
+  hipEvent_t timing_begin();
+  void timing_end(hipEvent_t e0);
 };
 
 }  // namespace ecg

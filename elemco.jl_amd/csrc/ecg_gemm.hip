@@ -7,6 +7,7 @@
 // MFMA operand layout of the 16x16x4 f64 builtin on gfx950 (checked with a lane probe):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16*(i%4), slot i/4.
 #include "ecg_gemm.h"
+#include "ecg_gemm_dev.h"
 
 #include <algorithm>
 
@@ -14,40 +15,22 @@ namespace ecg {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
+// K runs over the concatenation of nseg segments, each padded to a multiple of
+// the K slab (BKS) so that a slab never straddles two segments; slab k0 of the
+// padded range belongs to segment s with kp[s] <= k0 < kp[s+1].
 struct GemmParams {
-  int M, N, K, kchunk, ksplit, tiles_m, tiles_n;
-  const double* A; long a_s0, a_s1, a_k, a_z; int a_d0;
-  const double* B; long b_s0, b_s1, b_k, b_z; int b_d0;
+  int M, N, K, kchunk, ksplit, tiles_m, tiles_n;   // K: padded total
+  int nseg;
+  const double* sA[MAXSEG]; long sak[MAXSEG];
+  const double* sB[MAXSEG]; long sbk[MAXSEG];
+  int sK[MAXSEG], kp[MAXSEG + 1];
+  long a_s0, a_s1, a_z; int a_d0;
+  long b_s0, b_s1, b_z; int b_d0;
   double* C; long c_ms0, c_ms1, c_ns0, c_ns1, c_z; int c_md0, c_nd0;
   double alpha, beta;
   int cplain;   // 1: C is a plain column-major partial-sum slab (split-K workspace)
   long ldc_ws;  // leading dimension of the workspace slab
 };
-
-__device__ __forceinline__ long split_off(int x, int d0, long s0, long s1) {
-  int x1 = x / d0;
-  int x0 = x - x1 * d0;
-  return (long)x0 * s0 + (long)x1 * s1;
-}
-
-// Workgroup id -> (tile_m, tile_n).  Consecutive hardware ids are dealt
-// round-robin to the 8 XCDs; renumber so each XCD receives a contiguous range
-// of logical tiles, then walk tiles in groups of 8 M-tiles so neighbours on
-// one XCD share A and B panels in that XCD's L2.
-__device__ __forceinline__ void tile_coords(int wg, int nwg, int tiles_m, int& tm, int& tn) {
-  const int nx = 8;
-  int q = nwg / nx, r = nwg % nx;
-  int xcd = wg % nx, idx = wg / nx;
-  int logical = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + idx;
-  const int G = 8;
-  int tn_count = nwg / tiles_m;
-  int gid = logical / (G * tn_count);
-  int first_m = gid * G;
-  int gsize = min(tiles_m - first_m, G);
-  int in = logical - gid * G * tn_count;
-  tm = first_m + in % gsize;
-  tn = in / gsize;
-}
 
 constexpr int BM = 128, BN = 128, BKS = 16, LDP = BM + 16;
 
@@ -90,8 +73,6 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
   const int bz = z / p.ksplit, kz = z - bz * p.ksplit;
   const int kbeg = kz * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
-  const double* A = p.A + (long)bz * p.a_z;
-  const double* B = p.B + (long)bz * p.b_z;
 
   // per-thread row / column offsets (fixed along K)
   long aoff[8], boff[8];
@@ -126,20 +107,35 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
   }
 
   double ra[8], rb[8];
-  // branch-free: out-of-range elements read element 0 (always valid) and are zeroed
+  // branch-free: out-of-range elements read element 0 of the segment (always
+  // valid) and are zeroed.  The segment of slab k0 is uniform across the
+  // workgroup; selects with constant indices keep the kernel arguments in SGPRs.
   auto gload = [&](int k0) {
+    const double* A = p.sA[0];
+    const double* B = p.sB[0];
+    long a_k = p.sak[0], b_k = p.sbk[0];
+    int kl = k0, klen = p.sK[0];
+#pragma unroll
+    for (int q = 1; q < MAXSEG; ++q) {
+      if (q < p.nseg && k0 >= p.kp[q]) {
+        A = p.sA[q]; B = p.sB[q]; a_k = p.sak[q]; b_k = p.sbk[q];
+        kl = k0 - p.kp[q]; klen = p.sK[q];
+      }
+    }
+    A += (long)bz * p.a_z;
+    B += (long)bz * p.b_z;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      int k = AK ? (k0 + (t & 15)) : (k0 + (t >> 7) + 2 * j);
-      bool ok = aok[j] && k < kend;
-      double x = A[ok ? aoff[j] + (long)k * p.a_k : 0];
+      int k = AK ? (kl + (t & 15)) : (kl + (t >> 7) + 2 * j);
+      bool ok = aok[j] && k < klen;
+      double x = A[ok ? aoff[j] + (long)k * a_k : 0];
       ra[j] = ok ? x : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      int k = BK ? (k0 + (t & 15)) : (k0 + (t >> 7) + 2 * j);
-      bool ok = bok[j] && k < kend;
-      double x = B[ok ? boff[j] + (long)k * p.b_k : 0];
+      int k = BK ? (kl + (t & 15)) : (kl + (t >> 7) + 2 * j);
+      bool ok = bok[j] && k < klen;
+      double x = B[ok ? boff[j] + (long)k * b_k : 0];
       rb[j] = ok ? x : 0.0;
     }
   };
@@ -245,26 +241,51 @@ void GemmEngine::collect_timing() {
 
 void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
   if (op.M <= 0 || op.N <= 0 || op.batch <= 0) return;
-  ECG_CHECK(op.K > 0, "gemm: K must be positive");
-  ECG_CHECK(pr == Prec::F64 || pr == Prec::F32, "gemm: BF16 hi/lo products are not available in this build");
-  flops += 2LL * op.M * op.N * (long long)op.K * op.batch;
+  ECG_CHECK(op.nseg > 0 || op.K > 0, "gemm: K must be positive");
+  if (pr == Prec::BF16X3) {
+    // BF16 hi/lo products pay for packing the operands (8 bytes read and 4
+    // written per element) only when the output is large in both directions;
+    // skinny products (M N / (M + N) < 256) and batched ones use FP32 products.
+    const double mn = (double)op.M * op.N / ((double)op.M + op.N);
+    if (op.batch == 1 && mn >= 256.0) {
+      run_bf16x3(op);
+      return;
+    }
+    pr = Prec::F32;
+  }
+  ECG_CHECK(pr == Prec::F64 || pr == Prec::F32, "gemm: unknown precision");
   GemmParams p{};
-  p.M = op.M; p.N = op.N; p.K = op.K;
+  GemmSeg one;
+  one.A = op.A; one.a_k = op.a_k; one.B = op.B; one.b_k = op.b_k; one.K = op.K;
+  const GemmSeg* sg = op.nseg > 0 ? op.seg : &one;
+  p.nseg = op.nseg > 0 ? op.nseg : 1;
+  ECG_CHECK(p.nseg <= MAXSEG, "gemm: too many K segments");
+  long ktot = 0;
+  p.kp[0] = 0;
+  for (int s = 0; s < p.nseg; ++s) {
+    ECG_CHECK(sg[s].K > 0 && sg[s].A && sg[s].B, "gemm: empty K segment");
+    p.sA[s] = sg[s].A; p.sak[s] = sg[s].a_k; p.sB[s] = sg[s].B; p.sbk[s] = sg[s].b_k; p.sK[s] = sg[s].K;
+    p.kp[s + 1] = p.kp[s] + ((sg[s].K + BKS - 1) / BKS) * BKS;
+    ktot += sg[s].K;
+  }
+  for (int s = p.nseg; s < MAXSEG; ++s) { p.sA[s] = sg[0].A; p.sB[s] = sg[0].B; p.kp[s + 1] = p.kp[s]; }
+  flops += 2LL * op.M * op.N * ktot * op.batch;
+  p.M = op.M; p.N = op.N; p.K = p.kp[p.nseg];
   p.tiles_m = (op.M + BM - 1) / BM;
   p.tiles_n = (op.N + BN - 1) / BN;
-  p.A = op.A; p.a_s0 = op.a_s0; p.a_s1 = op.a_s1; p.a_k = op.a_k; p.a_z = op.a_z; p.a_d0 = op.a_d0;
-  p.B = op.B; p.b_s0 = op.b_s0; p.b_s1 = op.b_s1; p.b_k = op.b_k; p.b_z = op.b_z; p.b_d0 = op.b_d0;
+  p.a_s0 = op.a_s0; p.a_s1 = op.a_s1; p.a_z = op.a_z; p.a_d0 = op.a_d0;
+  p.b_s0 = op.b_s0; p.b_s1 = op.b_s1; p.b_z = op.b_z; p.b_d0 = op.b_d0;
   p.alpha = op.alpha; p.beta = op.beta;
   // split K when the output grid alone cannot fill 256 CUs
   long tiles = (long)p.tiles_m * p.tiles_n * op.batch;
   int ksplit = 1;
-  if (tiles < 512 && op.K >= 4 * BKS * 8) {
-    ksplit = (int)std::min<long>((1024 + tiles - 1) / tiles, op.K / (BKS * 8));
+  if (tiles < 512 && p.K >= 4 * BKS * 8) {
+    ksplit = (int)std::min<long>((1024 + tiles - 1) / tiles, p.K / (BKS * 8));
     ksplit = std::max(1, std::min(ksplit, 64));
   }
-  int kchunk = (op.K + ksplit - 1) / ksplit;
+  int kchunk = (p.K + ksplit - 1) / ksplit;
   kchunk = ((kchunk + BKS - 1) / BKS) * BKS;
-  ksplit = (op.K + kchunk - 1) / kchunk;
+  ksplit = (p.K + kchunk - 1) / kchunk;
   p.kchunk = kchunk; p.ksplit = ksplit;
   long slab = (long)op.M * op.N;
   if (ksplit > 1) {
@@ -278,19 +299,13 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
   if (ksplit > 1) { p.alpha = 1.0; p.beta = 0.0; }
   dim3 grid(p.tiles_m * p.tiles_n, 1, op.batch * ksplit);
   ECG_CHECK(grid.z <= 65535, "gemm: batch*ksplit exceeds grid limit");
-  const bool ak = (op.a_k == 1 && op.a_s0 != 1);
-  const bool bk = (op.b_k == 1 && op.b_s0 != 1);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (timed) {
-    while (evpool.size() < evused + 2) {
-      hipEvent_t e;
-      ECG_HIP(hipEventCreate(&e));
-      evpool.push_back(e);
-    }
-    e0 = evpool[evused++];
-    e1 = evpool[evused++];
-    ECG_HIP(hipEventRecord(e0, stream));
-  }
+  // K-contiguous operand loads; every segment must share the contiguity of the first
+  const bool ak = (sg[0].a_k == 1 && op.a_s0 != 1);
+  const bool bk = (sg[0].b_k == 1 && op.b_s0 != 1);
+  for (int s = 1; s < p.nseg; ++s)
+    ECG_CHECK((sg[s].a_k == 1 && op.a_s0 != 1) == ak && (sg[s].b_k == 1 && op.b_s0 != 1) == bk,
+              "gemm: K segments differ in operand contiguity");
+  hipEvent_t e0 = timing_begin();
   if (pr == Prec::F64) {
     if (!ak && !bk) gemm_kernel<double, false, false><<<grid, dim3(256), 0, stream>>>(p);
     else if (ak && !bk) gemm_kernel<double, true, false><<<grid, dim3(256), 0, stream>>>(p);
@@ -303,15 +318,35 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     else gemm_kernel<float, true, true><<<grid, dim3(256), 0, stream>>>(p);
   }
   ECG_HIP(hipGetLastError());
-  if (timed) ECG_HIP(hipEventRecord(e1, stream));
-  if (ksplit > 1) {
-    long total = slab;
-    dim3 g2((unsigned)((total + 255) / 256), op.batch);
-    hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, ws.d(), ksplit, slab, op.M, op.N,
-                       op.C, op.c_ms0, op.c_ms1, op.c_md0, op.c_ns0, op.c_ns1, op.c_nd0, op.c_z,
-                       op.alpha, op.beta);
-    ECG_HIP(hipGetLastError());
+  timing_end(e0);
+  if (ksplit > 1) reduce_slabs(op, ksplit);
+}
+
+// Event pair around one GEMM kernel launch when timing is on (pairs are read
+// back by collect_timing); returns the start event or null.
+hipEvent_t GemmEngine::timing_begin() {
+  if (!timed) return nullptr;
+  while (evpool.size() < evused + 2) {
+    hipEvent_t e;
+    ECG_HIP(hipEventCreate(&e));
+    evpool.push_back(e);
   }
+  hipEvent_t e0 = evpool[evused];
+  evused += 2;
+  ECG_HIP(hipEventRecord(e0, stream));
+  return e0;
+}
+
+void GemmEngine::timing_end(hipEvent_t e0) {
+  if (e0) ECG_HIP(hipEventRecord(evpool[evused - 1], stream));
+}
+
+void GemmEngine::reduce_slabs(const GemmOp& op, int ksplit) {
+  const long slab = (long)op.M * op.N;
+  dim3 g2((unsigned)((slab + 255) / 256), op.batch);
+  hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, ws.d(), ksplit, slab, op.M, op.N, op.C,
+                     op.c_ms0, op.c_ms1, op.c_md0, op.c_ns0, op.c_ns1, op.c_nd0, op.c_z, op.alpha, op.beta);
+  ECG_HIP(hipGetLastError());
 }
 
 }  // namespace ecg

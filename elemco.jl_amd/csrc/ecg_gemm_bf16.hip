@@ -1,0 +1,295 @@
+// BF16 hi/lo ("BF16x3") products on gfx950 MFMA.
+//
+// Every FP64 operand value x is split as x ~ h + l with h = bf16(x) and
+// l = bf16(x - h) (|x - h - l| <= 2^-17 |x| up to rounding), and a product is
+// formed from three BF16 MFMA terms h_a h_b + h_a l_b + l_a h_b accumulated in
+// FP32 (the l_a l_b term, ~2^-16 relative, is dropped).  C is updated in FP64.
+//
+// 1. pack_bf16x3_kernel turns one operand, given by the GEMM engine's
+//    split-stride (and segmented-K) addressing, into two planes P[r][k] of
+//    BF16 (k fastest), zero-padded to the tile multiples in r and to 32 in each
+//    K segment, through a 64x32 LDS transpose tile.
+// 2. gemm_bf16x3_kernel: 256x128 output tile per 512-thread workgroup (8 waves
+//    as 4x2, 64x64 per wave = 4x4 tiles of v_mfma_f32_16x16x32_bf16), K slabs of
+//    32 double-buffered in LDS (16-byte chunks, XOR-swizzled so a 16-lane row
+//    read touches every bank once), register prefetch of the next slab, FP64
+//    split-stride epilogue or split-K partial slabs.
+// MFMA operand layout (16x16x32 BF16, gfx950): A[i][k] in lane i + 16*(k/8),
+// element k%8; B[k][j] in lane j + 16*(k/8), element k%8; D[i][j] in lane
+// j + 16*(i/4), slot i%4.
+#include "ecg_gemm.h"
+#include "ecg_gemm_dev.h"
+
+#include <algorithm>
+
+namespace ecg {
+
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int XM = 256, XN = 128, XK = 32;   // output tile and K slab
+constexpr int PR = 64;                       // pack tile rows
+
+struct PackArgs {
+  int rows, kpad, nseg;
+  const double* src[MAXSEG];
+  long sk[MAXSEG];
+  int sK[MAXSEG], kp[MAXSEG + 1];
+  long s0, s1;
+  int d0;
+  u16* hi;   // [rows_pad][kpad]
+  u16* lo;
+};
+
+__device__ __forceinline__ u16 bf16_bits(float f) {
+  __bf16 b = (__bf16)f;
+  return __builtin_bit_cast(u16, b);
+}
+__device__ __forceinline__ float bf16_val(u16 u) { return (float)__builtin_bit_cast(__bf16, u); }
+
+// 1-D grid of (rows_pad / 64) * (kpad / 32) blocks (row tile fastest), 256 threads
+__global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p, int nrt) {
+  __shared__ u16 hs[XK][PR + 2];
+  __shared__ u16 ls[XK][PR + 2];
+  const int t = threadIdx.x;
+  const long bid = blockIdx.x;
+  const int r0 = (int)(bid % nrt) * PR, k0 = (int)(bid / nrt) * XK;
+  const double* src = p.src[0];
+  long sk = p.sk[0];
+  int kl0 = k0, klen = p.sK[0];
+#pragma unroll
+  for (int q = 1; q < MAXSEG; ++q)
+    if (q < p.nseg && k0 >= p.kp[q]) { src = p.src[q]; sk = p.sk[q]; kl0 = k0 - p.kp[q]; klen = p.sK[q]; }
+  // read: row r0 + (t % 64) (rows fastest: coalesced for row-contiguous sources), 8 k values each
+  const int row = t & (PR - 1);
+  const int r = r0 + row;
+  const bool rok = r < p.rows;
+  const long roff = rok ? split_off(r, p.d0, p.s0, p.s1) : 0;
+#pragma unroll
+  for (int j = 0; j < XK / 4; ++j) {
+    const int kk = (t >> 6) + 4 * j;
+    const int k = kl0 + kk;
+    const bool ok = rok && k < klen;
+    const double x = src[ok ? roff + (long)k * sk : 0];
+    const double v = ok ? x : 0.0;
+    const u16 h = bf16_bits((float)v);
+    const u16 l = bf16_bits((float)(v - (double)bf16_val(h)));
+    hs[kk][row] = h;
+    ls[kk][row] = l;
+  }
+  __syncthreads();
+  // write: row r0 + t/4, 8 consecutive k (one 16-byte chunk) per plane
+  const int wr = t >> 2, ch = t & 3;
+  u32x4_t vh, vl;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    vh[e] = (unsigned)hs[8 * ch + 2 * e][wr] | ((unsigned)hs[8 * ch + 2 * e + 1][wr] << 16);
+    vl[e] = (unsigned)ls[8 * ch + 2 * e][wr] | ((unsigned)ls[8 * ch + 2 * e + 1][wr] << 16);
+  }
+  const size_t off = (size_t)(r0 + wr) * p.kpad + k0 + 8 * ch;
+  *reinterpret_cast<u32x4_t*>(p.hi + off) = vh;
+  *reinterpret_cast<u32x4_t*>(p.lo + off) = vl;
+}
+
+struct X3Params {
+  int M, N, kpad, kchunk, ksplit, tiles_m, tiles_n;
+  const u16* Ah; const u16* Al;   // [tiles_m*256][kpad]
+  const u16* Bh; const u16* Bl;   // [tiles_n*128][kpad]
+  double* C; long c_ms0, c_ms1, c_ns0, c_ns1, c_z; int c_md0, c_nd0;
+  double alpha, beta;
+  int cplain; long ldc_ws;
+};
+
+__device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 3); }
+
+__global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
+  __shared__ u32x4_t As[2][2][XM][4];   // [buffer][plane][row][16-byte chunk]
+  __shared__ u32x4_t Bs[2][2][XN][4];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave & 3, wn = wave >> 2;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
+  const int bm = tm * XM, bn = tn * XN;
+  const int kz = blockIdx.z;
+  const int kbeg = kz * p.kchunk;
+  const int kend = min(p.kpad, kbeg + p.kchunk);
+  const size_t kp = (size_t)p.kpad;
+
+  // global chunks: A 2 planes x 256 rows x 4 chunks (4 per thread), B 2 x 128 x 4 (2 per thread)
+  u32x4_t ra[4], rb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int idx = t + 512 * j;
+      const int pl = idx >> 10, row = (idx >> 2) & (XM - 1), ch = idx & 3;
+      const u16* base = pl ? p.Al : p.Ah;
+      ra[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bm + row) * kp + k0 + 8 * ch);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int idx = t + 512 * j;
+      const int pl = idx >> 9, row = (idx >> 2) & (XN - 1), ch = idx & 3;
+      const u16* base = pl ? p.Bl : p.Bh;
+      rb[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bn + row) * kp + k0 + 8 * ch);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int idx = t + 512 * j;
+      const int pl = idx >> 10, row = (idx >> 2) & (XM - 1), ch = idx & 3;
+      As[buf][pl][row][swz(row, ch)] = ra[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int idx = t + 512 * j;
+      const int pl = idx >> 9, row = (idx >> 2) & (XN - 1), ch = idx & 3;
+      Bs[buf][pl][row][swz(row, ch)] = rb[j];
+    }
+  };
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int li = lane & 15, lk = lane >> 4;
+  int cur = 0;
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += XK) {
+    const bool more = k0 + XK < kend;
+    if (more) gload(k0 + XK);
+    bf16x8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = wm * 64 + i * 16 + li;
+      ah[i] = __builtin_bit_cast(bf16x8_t, As[cur][0][r][swz(r, lk)]);
+      al[i] = __builtin_bit_cast(bf16x8_t, As[cur][1][r][swz(r, lk)]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = wn * 64 + j * 16 + li;
+      bh[j] = __builtin_bit_cast(bf16x8_t, Bs[cur][0][c][swz(c, lk)]);
+      bl[j] = __builtin_bit_cast(bf16x8_t, Bs[cur][1][c][swz(c, lk)]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    if (more) sstore(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  // epilogue (FP64): D[i][j] of tile (i,j) sits in lane j + 16*(row/4), slot row%4
+  double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = bn + wn * 64 + j * 16 + li;
+    if (n >= p.N) continue;
+    const long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = bm + wm * 64 + i * 16 + 4 * lk + r;
+        if (m >= p.M) continue;
+        const long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
+        double v = p.alpha * (double)acc[i][j][r];
+        if (p.beta != 0.0) v += p.beta * C[cm + cn];
+        C[cm + cn] = v;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void GemmEngine::run_bf16x3(const GemmOp& op) {
+  ECG_CHECK(op.batch == 1, "gemm: batched BF16 hi/lo products are not supported");
+  GemmSeg one;
+  one.A = op.A; one.a_k = op.a_k; one.B = op.B; one.b_k = op.b_k; one.K = op.K;
+  const GemmSeg* sg = op.nseg > 0 ? op.seg : &one;
+  const int nseg = op.nseg > 0 ? op.nseg : 1;
+  ECG_CHECK(nseg <= MAXSEG, "gemm: too many K segments");
+  PackArgs pa{}, pb{};
+  pa.nseg = pb.nseg = nseg;
+  int kpad = 0;
+  long ktot = 0;
+  pa.kp[0] = pb.kp[0] = 0;
+  for (int s = 0; s < nseg; ++s) {
+    ECG_CHECK(sg[s].K > 0 && sg[s].A && sg[s].B, "gemm: empty K segment");
+    pa.src[s] = sg[s].A; pa.sk[s] = sg[s].a_k; pa.sK[s] = sg[s].K;
+    pb.src[s] = sg[s].B; pb.sk[s] = sg[s].b_k; pb.sK[s] = sg[s].K;
+    kpad += ((sg[s].K + XK - 1) / XK) * XK;
+    pa.kp[s + 1] = pb.kp[s + 1] = kpad;
+    ktot += sg[s].K;
+  }
+  for (int s = nseg; s < MAXSEG; ++s) {
+    pa.src[s] = pa.src[0]; pb.src[s] = pb.src[0];
+    pa.kp[s + 1] = pb.kp[s + 1] = kpad;
+  }
+  flops += 2LL * op.M * op.N * ktot;
+  const int tiles_m = (op.M + XM - 1) / XM, tiles_n = (op.N + XN - 1) / XN;
+  const long Mp = (long)tiles_m * XM, Np = (long)tiles_n * XN;
+  pk.ensure(sizeof(u16) * 2 * (size_t)(Mp + Np) * kpad);
+  u16* Ah = static_cast<u16*>(pk.p);
+  u16* Al = Ah + (size_t)Mp * kpad;
+  u16* Bh = Al + (size_t)Mp * kpad;
+  u16* Bl = Bh + (size_t)Np * kpad;
+  pa.rows = op.M; pa.kpad = kpad; pa.s0 = op.a_s0; pa.s1 = op.a_s1; pa.d0 = op.a_d0; pa.hi = Ah; pa.lo = Al;
+  pb.rows = op.N; pb.kpad = kpad; pb.s0 = op.b_s0; pb.s1 = op.b_s1; pb.d0 = op.b_d0; pb.hi = Bh; pb.lo = Bl;
+  const long nba = (Mp / PR) * (long)(kpad / XK), nbb = (Np / PR) * (long)(kpad / XK);
+  ECG_CHECK(nba < (1L << 31) && nbb < (1L << 31), "gemm: operand too large for the BF16 packing grid");
+  hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+  hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
+  ECG_HIP(hipGetLastError());
+
+  X3Params x{};
+  x.M = op.M; x.N = op.N; x.kpad = kpad; x.tiles_m = tiles_m; x.tiles_n = tiles_n;
+  x.Ah = Ah; x.Al = Al; x.Bh = Bh; x.Bl = Bl;
+  x.alpha = op.alpha; x.beta = op.beta;
+  // split K when the tiles alone cannot occupy the 256 CUs (one workgroup per CU)
+  const long tiles = (long)tiles_m * tiles_n;
+  int ksplit = 1;
+  if (tiles < 256 && kpad >= 8 * XK) {
+    ksplit = (int)std::min<long>((512 + tiles - 1) / tiles, kpad / (4 * XK));
+    ksplit = std::max(1, std::min(ksplit, 64));
+  }
+  int kchunk = (kpad + ksplit - 1) / ksplit;
+  kchunk = ((kchunk + XK - 1) / XK) * XK;
+  ksplit = (kpad + kchunk - 1) / kchunk;
+  x.kchunk = kchunk; x.ksplit = ksplit;
+  const long slab = (long)op.M * op.N;
+  if (ksplit > 1) {
+    ws.ensure(sizeof(double) * slab * ksplit);
+    x.C = ws.d(); x.cplain = 1; x.ldc_ws = op.M; x.c_z = slab;
+    x.alpha = 1.0; x.beta = 0.0;
+  } else {
+    x.C = op.C; x.cplain = 0;
+    x.c_ms0 = op.c_ms0; x.c_ms1 = op.c_ms1; x.c_md0 = op.c_md0;
+    x.c_ns0 = op.c_ns0; x.c_ns1 = op.c_ns1; x.c_nd0 = op.c_nd0;
+  }
+  ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
+  hipEvent_t e0 = timing_begin();
+  hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0, stream, x);
+  ECG_HIP(hipGetLastError());
+  timing_end(e0);
+  if (ksplit > 1) reduce_slabs(op, ksplit);
+}
+
+}  // namespace ecg

@@ -54,6 +54,20 @@ __global__ void mfma_probe_kernel(const double* a, const double* b, double* out,
   for (int r = 0; r < 4; ++r) outf[4 * l + r] = accf[r];
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// One wave: lane l supplies a[8l..8l+7] and b[8l..8l+7] (small integers, exact
+// in BF16) as the operand registers of the 16x16x32 BF16 MFMA and writes its
+// four FP32 accumulator slots to out[4l + r].
+__global__ void mfma_probe_bf16_kernel(const float* a, const float* b, float* out) {
+  int l = threadIdx.x;
+  bf16x8_t va, vb;
+  for (int t = 0; t < 8; ++t) { va[t] = (__bf16)a[8 * l + t]; vb[t] = (__bf16)b[8 * l + t]; }
+  f4_t acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[4 * l + r] = acc[r];
+}
+
 }  // namespace
 
 extern "C" {
@@ -107,6 +121,21 @@ int ecgt_mfma_probe(const double* a, const double* b, double* out, const float* 
     ECG_HIP(hipDeviceSynchronize());
     ECG_HIP(hipMemcpy(out, dout.p, 2048, hipMemcpyDeviceToHost));
     ECG_HIP(hipMemcpy(outf, doutf.p, 1024, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// Lane-layout probe of the 16x16x32 BF16 MFMA (64 lanes, 8 operands each, in bounds).
+int ecgt_mfma_probe_bf16(const float* a, const float* b, float* out) {
+  try {
+    ecg::DevBuf da(512 * 4), db(512 * 4), dout(256 * 4);
+    ECG_HIP(hipMemcpy(da.p, a, 2048, hipMemcpyHostToDevice));
+    ECG_HIP(hipMemcpy(db.p, b, 2048, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mfma_probe_bf16_kernel, dim3(1), dim3(64), 0, 0, da.f(), db.f(), dout.f());
+    ECG_HIP(hipDeviceSynchronize());
+    ECG_HIP(hipMemcpy(out, dout.p, 1024, hipMemcpyDeviceToHost));
     return 0;
   } catch (const std::exception&) {
     return -1;

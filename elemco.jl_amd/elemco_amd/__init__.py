@@ -4,7 +4,7 @@ The arithmetic lives in the HIP library ``libelemco_hip.so`` (C ABI declared in
 ``include/elemco_hip.h``); this package is the host-side mirror of the
 reference's Julia entry points.
 """
-from .lib import ElemcoError, load, Context, default_opts, ECG_CCSD, ECG_DCSD, ECG_FP64, ECG_FP32, ECG_BF16X3
+from .lib import ElemcoError, load, Context, default_opts, cc_solve_batch, ECG_CCSD, ECG_DCSD, ECG_FP64, ECG_FP32, ECG_BF16X3
 from .cc import (ECInfo, ccdriver, calc_MP2, calc_cc, calc_cc_resid, calc_pertT, calc_lm_cc,
                  calc_ccsd_vector_times_Jacobian, calc_1RDM, calc_fock_matrix)
 from .fcidump import read_fcidump, df_factors

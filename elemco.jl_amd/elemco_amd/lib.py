@@ -47,6 +47,8 @@ _SIGS = {
     "ecg_cc_state_step": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp],
     "ecg_cc_state_get": [_vp, _dp, _dp],
     "ecg_cc_solve": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _ip, _ip],
+    "ecg_cc_solve_batch": [ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts),
+                           ctypes.c_int, _dp, _ip, _ip, _ip],
     "ecg_pert_triples": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_pert_triples_range": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_long,
                                _dp, _dp],
@@ -247,6 +249,30 @@ class Context:
         self._chk(self._lib.ecg_profile(self._h, int(enable)))
 
     def stats(self):
-        s = np.zeros(5)
+        s = np.zeros(6)
         self._chk(self._lib.ecg_stats(self._h, _ptr(s)))
-        return dict(residual_ms=s[0], ladder_ms=s[1], gemm_ms=s[2], gemm_launches=int(s[3]), gemm_flops=s[4])
+        return dict(residual_ms=s[0], ladder_ms=s[1], gemm_ms=s[2], gemm_launches=int(s[3]), gemm_flops=s[4],
+                    fp64_iterations=int(s[5]))
+
+
+def cc_solve_batch(contexts, method=ECG_CCSD, precision=ECG_FP64, opts=None, max_threads=0):
+    """``ecg_cc_solve_batch``: solve independent systems concurrently.
+
+    Returns (energies [nsys, 6], iterations, converged); raises ElemcoError if any
+    system failed (the message names the first failing system).
+    """
+    lib = load()
+    opts = opts or default_opts()
+    n = len(contexts)
+    handles = (_vp * max(1, n))(*[c._h for c in contexts])
+    e = np.zeros((n, 6))
+    it = np.zeros(n, dtype=np.int32)
+    conv = np.zeros(n, dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    ip = lambda a: a.ctypes.data_as(_ip)
+    rc = lib.ecg_cc_solve_batch(n, handles, method, precision, ctypes.byref(opts), int(max_threads), _ptr(e), ip(it),
+                                ip(conv), ip(st))
+    if rc != 0:
+        k = int(np.nonzero(st)[0][0])
+        raise ElemcoError(f"system {k}: {lib.ecg_last_error(contexts[k]._h).decode()}")
+    return e, it.astype(int), conv.astype(bool)

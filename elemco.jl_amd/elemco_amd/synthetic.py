@@ -67,3 +67,22 @@ def make_system(nocc: int, nvirt: int, naux: int, seed: int, scale: float = 1.0)
     K = np.tensordot(B[:, :nocc, :], B[:nocc, :, :], axes=([1, 2], [0, 2]))
     h = np.diag(eps) - (2.0 * J - K)
     return eps, np.asfortranarray(h), B, 0.0
+
+
+BATCH_WIDTH = 0.7   # factor on conditioned_scale for the many-system batch
+
+
+def batch_sizes(nsys: int = 64, first_seed: int = 100):
+    """Sizes of the many-system batch: system k has seed first_seed + k,
+    nocc = 4 + floor(13 u) with u the first uniform draw of stream 4 of that seed
+    (nocc uniform in 4..16), nvirt = 9 nocc and naux = 3 (nocc + nvirt); the
+    factor width is 0.05 * BATCH_WIDTH * conditioned_scale (at the full
+    conditioned width some of the o=4 systems have divergent CCSD iterations).
+    Returns a list of (nocc, nvirt, naux, seed, scale)."""
+    out = []
+    for k in range(nsys):
+        seed = first_seed + k
+        o = 4 + int(13.0 * _uniform(seed, 4, 1)[0])
+        v, q = 9 * o, 3 * (o + 9 * o)
+        out.append((o, v, q, seed, BATCH_WIDTH * conditioned_scale(o, v, q)))
+    return out

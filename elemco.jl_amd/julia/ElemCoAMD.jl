@@ -154,7 +154,7 @@ end
 """
 `calc_cc(EC, method)` (src/cc/cc.jl:2451-2516) for "ccsd" / "dcsd": iterate on the
 device from the MP2 guess; returns `(energies, T1, T2)` with
-`energies = Dict("E" => E_H, "ESS", "EOS", "Eproj", "pESS", "pEOS")`.
+`energies = Dict("E" => E_H, "ESS", "EOS", "Eproj", "dE", "NormR")` of the last iteration.
 """
 function calc_cc(ctx::ECGContext, method::AbstractString; opts=default_opts(), precision=ECG_FP64)
   m = lowercase(method) == "dcsd" ? ECG_DCSD : ECG_CCSD
@@ -166,7 +166,7 @@ function calc_cc(ctx::ECGContext, method::AbstractString; opts=default_opts(), p
              ctx.handle, m, precision, opts, T1, T2, e, it, conv)
   _check(ctx, st)
   conv[] == 0 && @warn "CC iterations did not converge in $(it[]) iterations"
-  return Dict("E" => e[1], "ESS" => e[2], "EOS" => e[3], "Eproj" => e[4], "pESS" => e[5], "pEOS" => e[6]), T1, T2
+  return Dict("E" => e[1], "ESS" => e[2], "EOS" => e[3], "Eproj" => e[4], "dE" => e[5], "NormR" => e[6]), T1, T2
 end
 
 "`calc_lm_cc(EC, method)` (src/cc/cc_lagrange.jl:1120-1223): Λ multipliers `(U1, U2)` at amplitudes T1, T2."

@@ -72,18 +72,36 @@ int ecg_cc_residual_dev(ecg_ctx* ctx, int method, int precision, const double* d
 /* Device-resident iterations of cc_iterations! (src/cc/cc.jl:2518-2648).
  * state_init: amplitudes from T1/T2 (host), or the MP2 guess when both are NULL;
  * resets DIIS.  state_step: one iteration (residual, Hylleraas energy, update,
- * DIIS); info = [E_H, ESS, EOS, E_proj, dE, NormR, NormT, converged(0/1)].
+ * DIIS) in the given precision, without FP64 correction of a low-precision
+ * residual; info = [E_H, ESS, EOS, E_proj, dE, NormR, NormT, converged(0/1)].
  * state_get: copy the current amplitudes to the host. */
 int ecg_cc_state_init(ecg_ctx* ctx, const double* T1, const double* T2, const ecg_solver_opts* opts);
 int ecg_cc_state_step(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, double* info);
 int ecg_cc_state_get(ecg_ctx* ctx, double* T1, double* T2);
 
 /* calc_cc for the closed-shell path (src/cc/cc.jl:2451-2516): iterate from the
- * MP2 guess (T1 = T2 = NULL on input) or from the given amplitudes until
- * converged.  energies = [E_H, ESS, EOS, E_proj, pESS, pEOS]; T1/T2 receive the
- * converged amplitudes when not NULL. */
+ * MP2 guess until converged.  energies = [E_H, ESS, EOS, E_proj, dE, NormR] of
+ * the last iteration; T1/T2 receive the converged amplitudes when not NULL.
+ * precision = ECG_FP32 (mixed, defect correction): iterations use the residual
+ * R_low(T) + delta with FP32 products; delta = R_64 - R_low is refreshed by an
+ * FP64 residual at the current T every opts->refine_every-th iteration (when
+ * > 0) and whenever the corrected NormR drops below 100*thr.  Convergence (the
+ * reference's test) is only declared on such an exact iteration, so the result
+ * is the FP64 solution; ecg_stats reports how many FP64 residuals were used. */
 int ecg_cc_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, double* T1,
                  double* T2, double* energies, int* iterations, int* converged);
+
+/* Many independent systems: ecg_cc_solve for each of nsys contexts (created
+ * on the same or different devices), run concurrently -- one host thread and
+ * the context's own HIP stream per system, at most max_threads at a time
+ * (0: min(nsys, 16)).  The reference drives one system per process
+ * (ccdriver, src/cc/ccdriver.jl:37-76); this is its batched form.  Outputs
+ * are per system: energies 6*nsys (layout of ecg_cc_solve), iterations and
+ * converged nsys each.  status[k] (nsys, may be NULL) receives each system's
+ * status; the return value is 0 if all succeeded, else the first failure's
+ * status (its message is in ecg_last_error of that context). */
+int ecg_cc_solve_batch(int nsys, ecg_ctx* const* ctxs, int method, int precision, const ecg_solver_opts* opts,
+                       int max_threads, double* energies, int* iterations, int* converged, int* status);
 
 /* Perturbative triples for the closed-shell path.  lambda = 0:
  * calc_pertT_closed_shell (src/cc/cc_triples.jl:39-148) with amplitudes T1, T2;
@@ -117,7 +135,7 @@ int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* 
  * print_time, src/tools/utils.jl:51-57): enable = 1 records HIP events around
  * every MFMA GEMM launch on the library stream and zeroes the counters.
  * stats = [last residual ms, last ladder ms, timed GEMM ms, timed GEMM launches,
- *          issued GEMM flops] */
+ *          issued GEMM flops, FP64 iterations of the last ecg_cc_solve] (6 doubles) */
 int ecg_profile(ecg_ctx* ctx, int enable);
 int ecg_stats(ecg_ctx* ctx, double* stats);
 

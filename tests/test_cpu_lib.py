@@ -27,6 +27,17 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+HOOKS = os.path.join(ROOT, "elemco.jl_amd", "libecg_testhooks.so")
+
+
+@pytest.mark.skipif(not os.path.exists(HOOKS), reason="test hooks not built")
+def test_test_hook_library_resolves_all_symbols():
+    """The GPU tests' hook library loads (every symbol resolved at load time)."""
+    h = ctypes.CDLL(HOOKS, mode=os.RTLD_NOW)
+    for s in ("ecgt_contract", "ecgt_permute", "ecgt_bench_contract", "ecgt_mfma_probe", "ecgt_mfma_probe_bf16"):
+        assert hasattr(h, s), s
+
+
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
 def test_python_binding_covers_header():
     from elemco_amd import lib

@@ -80,6 +80,26 @@ def test_solve_c0(method):
     assert abs(e["HF"] - eo["HF"]) < 1e-10
 
 
+@pytest.mark.parametrize("method", [E.ECG_CCSD, E.ECG_DCSD])
+@pytest.mark.parametrize("prec,refine_every", [(E.ECG_FP32, 0), (E.ECG_FP32, 4), (E.ECG_BF16X3, 0)])
+def test_mixed_solve_converges_to_fp64(method, prec, refine_every):
+    """Defect-corrected low-precision solve: same converged energy as FP64 (1e-9 Eh,
+    both converged to thr = 1e-13 so that the comparison does not depend on the
+    trajectory), convergence declared on an FP64 iteration, few FP64 residuals."""
+    from elemco_amd.synthetic import conditioned_scale
+    # BF16 hi/lo products are used only for large products (M N / (M + N) >= 256)
+    no, nv, nq, seed = (6, 40, 90, 21) if prec == E.ECG_FP32 else (10, 90, 300, 5)
+    EC = E.ECInfo.synthetic(no, nv, nq, seed, scale=0.7 * conditioned_scale(no, nv, nq))
+    opts = E.default_opts(maxit=80, refine_every=refine_every, thr=1e-13)
+    e64, _, _, it64, c64 = EC.ctx.cc_solve(method, E.ECG_FP64, opts)
+    e32, _, _, it32, c32 = EC.ctx.cc_solve(method, prec, opts)
+    n64 = EC.ctx.stats()["fp64_iterations"]
+    assert c64 and c32
+    assert abs(float(e32[0]) - float(e64[0])) < 1e-9
+    assert 1 <= n64 < it32
+    assert float(e32[5]) < 1e-13      # the final (exact) residual norm meets thr
+
+
 def test_h2o_fcidump_ccsd():
     """Reference fixture test/files/H2O.FCIDUMP through the DF factorisation (test/h2o.jl:6-8)."""
     EC = E.ECInfo.from_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"))

@@ -82,3 +82,24 @@ def test_contract_fp32_products(la, lb, lc):
     # one rounding of each operand to FP32 plus FP32 accumulation, against the FP64 result
     bound = 4.0 * np.sqrt(K) * 2.0**-23 * np.einsum(f"{la},{lb}->{lc}", np.abs(A), np.abs(B)).max()
     assert float(np.max(np.abs(got - ref))) < bound
+
+
+@pytest.mark.parametrize("la,lb,lc", [("akic", "cbkj", "abij"), ("klcd", "cdij", "klij"), ("ad", "dbij", "abij"),
+                                      ("aiQ", "kcQ", "akic")])
+def test_contract_bf16x3_products(la, lb, lc):
+    """BF16 hi/lo products (three BF16 MFMA terms, FP32 accumulation).
+
+    Each operand is represented to 2^-17 relative and the dropped lo*lo term is
+    below 2^-16 relative, so |error| <= (2^-14 + 4 sqrt(K) 2^-23) * (|A||B|);
+    a lost hi*lo term (error ~2^-9 relative) would fail this bound."""
+    ext = SIZES["ragged"]
+    rng = np.random.default_rng(19)
+    A, B, C = _rand(la, ext, rng), _rand(lb, ext, rng), _rand(lc, ext, rng)
+    ref = np.einsum(f"{la},{lb}->{lc}", A, B) + C
+    got = gpu_contract(1.0, A, la, B, lb, 1.0, C, lc, prec=2)
+    K = np.prod([ext[c] for c in la if c not in lc])
+    scale = np.einsum(f"{la},{lb}->{lc}", np.abs(A), np.abs(B)).max()
+    bound = (2.0**-14 + 4.0 * np.sqrt(K) * 2.0**-23) * scale
+    err = float(np.max(np.abs(got - ref)))
+    assert err < bound
+    assert err > 0.0          # the path is not silently FP64
