@@ -8,8 +8,10 @@ DIIS) with integrals and amplitudes resident in HBM before the timed region.
 
 Prints one JSON line.  value = algorithmic TFLOP/s of the FP64 CCSD iteration
 (count in elemco_amd/flops.py); ms_per_step = FP64 time per iteration.  The
-same line carries the mixed-precision (FP32-MFMA) iteration, (T) rates on a
-sample of triplets, the roofline of the dominant kernel and the CPU baseline.
+same line carries the mixed-precision iterations (FP32 and BF16 hi/lo
+products), full FP64 and mixed solves with their energy difference, (T) rates
+on a sample of triplets, the roofline of the dominant kernel and the CPU
+baseline.
 """
 import argparse
 import json
@@ -128,8 +130,8 @@ def main():
     ap.add_argument("--triplets", type=int, default=48, help="(T) sample size; 0 skips (T)")
     ap.add_argument("--no-mixed", action="store_true", help="time only the FP64 iteration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--solve", action="store_true",
-                    help="also run full FP64 and mixed solves on the conditioned system of the same sizes")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the full FP64 / mixed solves on the conditioned system of the same sizes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -189,7 +191,7 @@ def main():
         if not args.no_mixed:
             line["triples"]["fp32"] = time_triples(E, ctx, E.ECG_FP32, args.triplets, o, v)
             line["triples"]["bf16x3"] = time_triples(E, ctx, E.ECG_BF16X3, args.triplets, o, v)
-    if args.solve:
+    if not args.no_solve:
         log("full solves, FP64 and mixed")
         ctx.close()
         line["solve"] = solve_study(E, o, v, q, seed, meth)

@@ -107,9 +107,11 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
   }
 
   double ra[8], rb[8];
+  unsigned amask = 0, bmask = 0;   // bit j: element j of the prefetched slab is in range
   // branch-free: out-of-range elements read element 0 of the segment (always
-  // valid) and are zeroed.  The segment of slab k0 is uniform across the
-  // workgroup; selects with constant indices keep the kernel arguments in SGPRs.
+  // valid) and are zeroed when stored to LDS.  The segment of slab k0 is uniform
+  // across the workgroup; selects with constant indices keep the kernel
+  // arguments in SGPRs.
   auto gload = [&](int k0) {
     const double* A = p.sA[0];
     const double* B = p.sB[0];
@@ -124,31 +126,40 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
     }
     A += (long)bz * p.a_z;
     B += (long)bz * p.b_z;
+    // one 64-bit product per operand and slab: rows (AK) or k (otherwise) advance by a fixed stride
+    const int kta = AK ? (kl + (t & 15)) : (kl + (t >> 7));
+    const int ktb = BK ? (kl + (t & 15)) : (kl + (t >> 7));
+    const long ka = (long)kta * a_k, kb = (long)ktb * b_k;
+    amask = bmask = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      int k = AK ? (kl + (t & 15)) : (kl + (t >> 7) + 2 * j);
-      bool ok = aok[j] && k < klen;
-      double x = A[ok ? aoff[j] + (long)k * a_k : 0];
-      ra[j] = ok ? x : 0.0;
+      const int k = AK ? kta : kta + 2 * j;
+      const bool ok = aok[j] && k < klen;
+      const long off = AK ? aoff[j] + ka : aoff[0] + ka + (long)(2 * j) * a_k;
+      ra[j] = A[ok ? off : 0];
+      amask |= (unsigned)ok << j;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      int k = BK ? (kl + (t & 15)) : (kl + (t >> 7) + 2 * j);
-      bool ok = bok[j] && k < klen;
-      double x = B[ok ? boff[j] + (long)k * b_k : 0];
-      rb[j] = ok ? x : 0.0;
+      const int k = BK ? ktb : ktb + 2 * j;
+      const bool ok = bok[j] && k < klen;
+      const long off = BK ? boff[j] + kb : boff[0] + kb + (long)(2 * j) * b_k;
+      rb[j] = B[ok ? off : 0];
+      bmask |= (unsigned)ok << j;
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (AK) As[buf][t & 15][(t >> 4) + 16 * j] = (T)ra[j];
-      else    As[buf][(t >> 7) + 2 * j][t & 127] = (T)ra[j];
+      const T x = ((amask >> j) & 1) ? (T)ra[j] : (T)0;
+      if (AK) As[buf][t & 15][(t >> 4) + 16 * j] = x;
+      else    As[buf][(t >> 7) + 2 * j][t & 127] = x;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (BK) Bs[buf][t & 15][(t >> 4) + 16 * j] = (T)rb[j];
-      else    Bs[buf][(t >> 7) + 2 * j][t & 127] = (T)rb[j];
+      const T x = ((bmask >> j) & 1) ? (T)rb[j] : (T)0;
+      if (BK) Bs[buf][t & 15][(t >> 4) + 16 * j] = x;
+      else    Bs[buf][(t >> 7) + 2 * j][t & 127] = x;
     }
   };
 
@@ -167,7 +178,10 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
   __syncthreads();
   for (int k0 = kbeg; k0 < kend; k0 += BKS) {
     const bool more = k0 + BKS < kend;
-    if (more) gload(k0 + BKS);
+    // unconditional prefetch (the last slab reloads itself): a conditional load would make
+    // the compiler wait for it before the MFMAs below instead of before the LDS store
+    gload(more ? k0 + BKS : k0);
+    asm volatile("" ::: "memory");   // keep the prefetch loads issued ahead of the MFMAs
 #pragma unroll
     for (int kk = 0; kk < BKS; kk += 4) {
       T a[4], b[4];

@@ -168,7 +168,10 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   __syncthreads();
   for (int k0 = kbeg; k0 < kend; k0 += XK) {
     const bool more = k0 + XK < kend;
-    if (more) gload(k0 + XK);
+    // unconditional prefetch (the last slab reloads itself): a conditional load would make
+    // the compiler wait for it before the MFMAs below instead of before the LDS store
+    gload(more ? k0 + XK : k0);
+    asm volatile("" ::: "memory");   // keep the prefetch loads issued ahead of the MFMAs
     bf16x8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

@@ -51,27 +51,30 @@ struct CombArgs {
 // once per triplet instead of six times.
 __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
   __shared__ double box[6][TT * TT * TT];
-  __shared__ double red[6][TT * TT * TT];
+  __shared__ double wsh[6][TT * TT];    // <jk|bc>-type slices (0-2) and Y slices (3-5) of the block
+  __shared__ double esh[3][TT];         // virtual orbital energies of the block's a, b, c tiles
+  __shared__ double pab[6][4][TT];      // per-wave partials of the a- and b-indexed vectors
+  __shared__ double pcc[6][TT];         // c-indexed vectors
   __shared__ double ered[4];
   const int nt = p.ntile;
   const long v = p.nv;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int la = lane & 7, lb = lane >> 3;   // element e = la + 8 lb + 64 lc, lc = w and w + 4
   const int T[3] = {p.tri[3 * blockIdx.x], p.tri[3 * blockIdx.x + 1], p.tri[3 * blockIdx.x + 2]};
   // permutation table: which of the three coordinates sits at K index 1,2,3
   const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}, {2, 0, 1}, {1, 2, 0}};
   // slot s holds K at tile coordinates (T[perm[s][0]], T[perm[s][1]], T[perm[s][2]])
   for (int s = 0; s < 6; ++s) {
     const int b1 = T[perm[s][0]] * TT, b2 = T[perm[s][1]] * TT, b3 = T[perm[s][2]] * TT;
-    for (int e = threadIdx.x; e < TT * TT * TT; e += blockDim.x) {
+    for (int e = t; e < TT * TT * TT; e += blockDim.x) {
       int x1 = b1 + e % TT, x2 = b2 + (e / TT) % TT, x3 = b3 + e / (TT * TT);
       bool ok = x1 < v && x2 < v && x3 < v;
       box[s][e] = ok ? p.K[(long)x1 + v * x2 + v * v * x3] : 0.0;
     }
   }
-  __syncthreads();
   const double eijk = p.eo[p.i] + p.eo[p.j] + p.eo[p.k];
-  const long ooff_jk = v * v * (p.j + (long)p.no * p.k);
-  const long ooff_ik = v * v * (p.i + (long)p.no * p.k);
-  const long ooff_ij = v * v * (p.i + (long)p.no * p.j);
+  const long ooff[3] = {v * v * (p.j + (long)p.no * p.k), v * v * (p.i + (long)p.no * p.k),
+                        v * v * (p.i + (long)p.no * p.j)};
   for (int o = 0; o < 6; ++o) {
     // output block tile coordinates; skip permutations equal to an earlier one (ties)
     const int tg[3] = {T[perm[o][0]], T[perm[o][1]], T[perm[o][2]]};
@@ -79,6 +82,22 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
     for (int o2 = 0; o2 < o; ++o2)
       dup = dup || (T[perm[o2][0]] == tg[0] && T[perm[o2][1]] == tg[1] && T[perm[o2][2]] == tg[2]);
     if (dup) continue;
+    const int base[3] = {tg[0] * TT, tg[1] * TT, tg[2] * TT};
+    // stage the 8x8 slices of W and Y this block multiplies X with, and the orbital energies:
+    // slice q%3 = 0 over (b,c) at (j,k), 1 over (a,c) at (i,k), 2 over (a,b) at (i,j)
+    for (int u = t; u < 6 * TT * TT; u += blockDim.x) {
+      const int q = u / (TT * TT), x = u % TT, y = (u / TT) % TT;
+      const int role = q % 3;
+      const int d1 = role == 0 ? 1 : 0, d2 = role == 2 ? 1 : 2;   // the two coordinates of the slice
+      const int c1 = base[d1] + x, c2 = base[d2] + y;
+      const double* src = q < 3 ? p.W : p.Y;
+      const bool ok = c1 < v && c2 < v;
+      wsh[q][x + TT * y] = ok ? src[c1 + v * c2 + ooff[role]] : 0.0;
+    }
+    if (t < 3 * TT) {
+      const int d = t / TT, x = base[d] + t % TT;
+      esh[d][t % TT] = x < v ? p.ev[x] : 0.0;
+    }
     // the box holding permutation q of this block's coordinates
     int slot[6];
 #pragma unroll
@@ -89,12 +108,15 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
         if (T[perm[s][0]] == w0 && T[perm[s][1]] == w1 && T[perm[s][2]] == w2) sl = s;
       slot[q] = sl;
     }
-    const int a0 = tg[0] * TT, b0 = tg[1] * TT, c0 = tg[2] * TT;
+    __syncthreads();   // boxes (first block) and slices staged
     double esum = 0.0;
-    for (int e = threadIdx.x; e < TT * TT * TT; e += blockDim.x) {
-      int la = e % TT, lb = (e / TT) % TT, lc = e / (TT * TT);
-      int a = a0 + la, b = b0 + lb, c = c0 + lc;
-      double r[6] = {0, 0, 0, 0, 0, 0};
+    double rs[2][6];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int lc = w + 4 * h;
+      const int a = base[0] + la, b = base[1] + lb, c = base[2] + lc;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) rs[h][q] = 0.0;
       if (a < v && b < v && c < v) {
         // local coordinates (la,lb,lc) read in the order of each permutation
         const int lo[3] = {la, lb, lc};
@@ -102,42 +124,54 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
 #pragma unroll
         for (int q = 0; q < 6; ++q)
           kv[q] = box[slot[q]][lo[perm[q][0]] + TT * lo[perm[q][1]] + TT * TT * lo[perm[q][2]]];
-        double x = (4.0 * kv[0] - 2.0 * kv[1] - 2.0 * kv[2] - 2.0 * kv[3] + kv[4] + kv[5]) /
-                   (eijk - p.ev[a] - p.ev[b] - p.ev[c]);
-        double kk = p.KU ? p.KU[(long)a + v * b + v * v * c] : kv[0];
+        const double x = (4.0 * kv[0] - 2.0 * kv[1] - 2.0 * kv[2] - 2.0 * kv[3] + kv[4] + kv[5]) /
+                         (eijk - esh[0][la] - esh[1][lb] - esh[2][lc]);
+        const double kk = p.KU ? p.KU[(long)a + v * b + v * v * c] : kv[0];
         esum += kk * x;
-        r[0] = x * p.W[b + v * c + ooff_jk];   // IntX[a,i]
-        r[1] = x * p.W[a + v * c + ooff_ik];   // IntX[b,j]
-        r[2] = x * p.W[a + v * b + ooff_ij];   // IntX[c,k]
-        r[3] = x * p.Y[b + v * c + ooff_jk];   // IntY[a,i]
-        r[4] = x * p.Y[a + v * c + ooff_ik];   // IntY[b,j]
-        r[5] = x * p.Y[a + v * b + ooff_ij];   // IntY[c,k]
+        rs[h][0] = x * wsh[0][lb + TT * lc];   // IntX[a,i]
+        rs[h][1] = x * wsh[1][la + TT * lc];   // IntX[b,j]
+        rs[h][2] = x * wsh[2][la + TT * lb];   // IntX[c,k]
+        rs[h][3] = x * wsh[3][lb + TT * lc];   // IntY[a,i]
+        rs[h][4] = x * wsh[4][la + TT * lc];   // IntY[b,j]
+        rs[h][5] = x * wsh[5][la + TT * lb];   // IntY[c,k]
       }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) red[q][e] = r[q];
     }
-    // energy: wavefront reduction, then across the four waves
-    for (int s = 32; s > 0; s >>= 1) esum += __shfl_down(esum, s, 64);
-    if ((threadIdx.x & 63) == 0) ered[threadIdx.x >> 6] = esum;
+    // fixed-order reductions: a-indexed vectors over (lb, lc), b-indexed over (la, lc),
+    // c-indexed over (la, lb); lanes by xor trees, waves through LDS
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int role = q % 3;
+      if (role == 0) {
+        double s = rs[0][q] + rs[1][q];
+        s += __shfl_xor(s, 8, 64);
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (lb == 0) pab[q][w][la] = s;
+      } else if (role == 1) {
+        double s = rs[0][q] + rs[1][q];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
+        if (la == 0) pab[q][w][lb] = s;
+      } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          double s = rs[h][q];
+          for (int m = 1; m < 64; m <<= 1) s += __shfl_xor(s, m, 64);
+          if (lane == 0) pcc[q][w + 4 * h] = s;
+        }
+      }
+    }
+    for (int m = 1; m < 64; m <<= 1) esum += __shfl_xor(esum, m, 64);
+    if (lane == 0) ered[w] = esum;
     __syncthreads();
     double* out = p.part + ((long)tg[0] + (long)nt * (tg[1] + (long)nt * tg[2])) * NPART;
-    if (threadIdx.x < 48) {
-      int q = threadIdx.x / TT, l = threadIdx.x % TT;
-      // vectors 0,3 are indexed by a (sum over lb,lc); 1,4 by b; 2,5 by c
-      int role = q % 3;
-      double s = 0.0;
-      for (int x = 0; x < TT; ++x)
-        for (int y = 0; y < TT; ++y) {
-          int la, lb, lc;
-          if (role == 0) { la = l; lb = x; lc = y; }
-          else if (role == 1) { lb = l; la = x; lc = y; }
-          else { lc = l; la = x; lb = y; }
-          s += red[q][la + TT * lb + TT * TT * lc];
-        }
-      out[threadIdx.x] = s;
+    if (t < 6 * TT) {
+      const int q = t / TT, l = t % TT;
+      out[t] = (q % 3 == 2) ? pcc[q][l] : ((pab[q][0][l] + pab[q][1][l]) + (pab[q][2][l] + pab[q][3][l]));
     }
-    if (threadIdx.x == 0) out[48] = ered[0] + ered[1] + ered[2] + ered[3];
-    __syncthreads();   // red and ered are reused by the next output block
+    if (t == 0) out[48] = (ered[0] + ered[1]) + (ered[2] + ered[3]);
+    __syncthreads();   // slices and partial buffers are reused by the next output block
   }
 }
 

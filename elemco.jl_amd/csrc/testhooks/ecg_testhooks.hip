@@ -77,6 +77,11 @@ __global__ void fill_pattern_kernel(double* p, long n, double s) {
   for (; i < n; i += (long)gridDim.x * blockDim.x) p[i] = s * (double)((i * 2654435761L) % 1000 - 500) * 1e-3;
 }
 
+__global__ void copy_kernel(const double* __restrict__ x, double* __restrict__ y, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
 // Time `reps` evaluations of one contraction on device-resident operands; returns ms per call.
 double ecgt_bench_contract(int prec, int na, const long* adims, const char* la, int nb, const long* bdims,
                            const char* lb, int nc, const long* cdims, const char* lc, int reps) {
@@ -124,6 +129,31 @@ int ecgt_mfma_probe(const double* a, const double* b, double* out, const float* 
     return 0;
   } catch (const std::exception&) {
     return -1;
+  }
+}
+
+// Plain device copy of n doubles (reads 8n bytes, writes 8n bytes), the
+// calibration kernel for the FETCH_SIZE / WRITE_SIZE counters; returns ms per copy.
+double ecgt_copy_bench(long n, int reps) {
+  try {
+    Env& e = env();
+    ecg::DevBuf x(sizeof(double) * n), y(sizeof(double) * n);
+    hipLaunchKernelGGL(fill_pattern_kernel, dim3(4096), dim3(256), 0, e.s, x.d(), n, 1.0);
+    hipEvent_t a, b;
+    ECG_HIP(hipEventCreate(&a));
+    ECG_HIP(hipEventCreate(&b));
+    ECG_HIP(hipEventRecord(a, e.s));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e.s, x.d(), y.d(), n);
+    ECG_HIP(hipEventRecord(b, e.s));
+    ECG_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    ECG_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+  } catch (const std::exception&) {
+    return -1.0;
   }
 }
 

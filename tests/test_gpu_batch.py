@@ -44,6 +44,8 @@ def test_batch_matches_single_solves_and_oracle():
     k = int(np.argmin([s[0] for s in sizes]))
     o, v, q, seed, sc = sizes[k]
     ECo = O.ECInfo.from_df(make_df_system(o, v, q, seed, sc))
+    ECo.options["thr"] = 1e-12
+    ECo.options["maxit"] = 100
     O.calc_fock_matrix(ECo)
     O.calc_MP2(ECo)
     Eo, conv, _ = O.calc_cc(ECo, dc=False)

@@ -141,3 +141,21 @@ def test_residual_c1_golden(method):
     assert abs(np.sum(R2 * w) - chk["weighted"]) < 1e-9
     for idx, val in chk["samples"]:
         assert abs(R2[tuple(idx)] - val) < 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(C1_GOLDEN), reason="c1 golden vectors not generated")
+def test_conditioned_c1_energies_and_triples_golden():
+    """o=20, v=180, naux=600 (factor width scaled by conditioned_scale): converged
+    DCSD and CCSD energies and CCSD(T) against the committed oracle values."""
+    g = json.load(open(C1_GOLDEN))
+    c, cond = g["config"], g["conditioned"]
+    EC = E.ECInfo.synthetic(c["nocc"], c["nvirt"], c["naux"], c["seed"], scale=cond["scale"])
+    _, _, ehf = EC.ctx.fock()
+    assert abs(ehf - cond["HF"]) < 1e-9
+    for meth in ("dcsd", "ccsd"):
+        e = E.ccdriver(EC, meth)
+        assert e["converged"]
+        assert abs(e[meth.upper() + "c"] - cond[meth.upper() + "c"]) < 1e-9
+    et, etb = EC.ctx.pert_triples(EC.files["T_vo"], EC.files["T_vvoo"])
+    assert abs(et - cond["(T)"]) < 1e-9
+    assert abs(etb - cond["[T]"]) < 1e-9
