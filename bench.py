@@ -102,6 +102,27 @@ def solve_study(E, o, v, q, seed, meth, maxit=60):
     return out
 
 
+def engine_bench(n=8192):
+    """GEMM-engine rates on a plain n^3 product per precision, and a 1 GiB device copy
+    (the calibration kernel for HBM byte counters); uses the engine's test-hook library."""
+    import ctypes
+    h = ctypes.CDLL(os.path.join(ROOT, "elemco.jl_amd", "libecg_testhooks.so"))
+    lp = ctypes.POINTER(ctypes.c_long)
+    h.ecgt_bench_contract.restype = ctypes.c_double
+    h.ecgt_bench_contract.argtypes = [ctypes.c_int, ctypes.c_int, lp, ctypes.c_char_p, ctypes.c_int, lp,
+                                      ctypes.c_char_p, ctypes.c_int, lp, ctypes.c_char_p, ctypes.c_int]
+    h.ecgt_copy_bench.restype = ctypes.c_double
+    h.ecgt_copy_bench.argtypes = [ctypes.c_long, ctypes.c_int]
+    dims = (ctypes.c_long * 2)(n, n)
+    out = {"gemm_n": n}
+    for name, prec in (("fp64", 0), ("fp32", 1), ("bf16x3", 2)):
+        ms = h.ecgt_bench_contract(prec, 2, dims, b"ik", 2, dims, b"kj", 2, dims, b"ij", 1)
+        out[name] = {"ms": round(ms, 3), "tflops": round(2.0 * n**3 / ms / 1e9, 2)}
+    ms = h.ecgt_copy_bench(1 << 27, 1)
+    out["copy_1GiB"] = {"ms": round(ms, 3), "GB_per_s": round(2 * 2**30 / ms / 1e6, 1)}
+    return out
+
+
 def time_triples(E, ctx, prec, nsample, o, v):
     """(T) rate from triplets 8 .. 8+nsample-1: time(0, 8+n) - time(0, 8) excludes the integral set-up."""
     import numpy as np
@@ -130,6 +151,8 @@ def main():
     ap.add_argument("--triplets", type=int, default=48, help="(T) sample size; 0 skips (T)")
     ap.add_argument("--no-mixed", action="store_true", help="time only the FP64 iteration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--engine", action="store_true",
+                    help="only the GEMM-engine rates on 8192^3 and a 1 GiB copy (one JSON object)")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the full FP64 / mixed solves on the conditioned system of the same sizes")
     args = ap.parse_args()
@@ -137,6 +160,10 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.engine:
+        print(json.dumps(engine_bench()), flush=True)
+        return
 
     import elemco_amd as E
     from elemco_amd.flops import ccsd_iteration

@@ -291,6 +291,7 @@ int ecg_profile(ecg_ctx* ctx, int enable) {
     c.ops.gemm.timed_ms = 0.0;
     c.ops.gemm.timed_launches = 0;
     c.ops.gemm.flops = 0;
+    c.ops.gemm.bytes = 0.0;
   });
 }
 
@@ -305,6 +306,7 @@ int ecg_stats(ecg_ctx* ctx, double* stats) {
     stats[3] = (double)c.ops.gemm.timed_launches;
     stats[4] = (double)c.ops.gemm.flops;
     stats[5] = (double)c.fp64_iterations;
+    stats[6] = c.ops.gemm.bytes;
   });
 }
 

@@ -62,6 +62,7 @@ struct GemmEngine {
   DevBuf ws;          // split-K partial sums
   Prec prec = Prec::F64;
   long long flops = 0;   // running count of 2*M*N*K issued (for reporting)
+  double bytes = 0.0;    // running count of algorithmic bytes: 8 (M K + K N) + 8 M N (16 M N when beta != 0)
   // optional per-launch timing with HIP events on `stream` (off by default)
   bool timed = false;
   double timed_ms = 0.0;

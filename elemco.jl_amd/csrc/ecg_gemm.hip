@@ -284,6 +284,7 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
   }
   for (int s = p.nseg; s < MAXSEG; ++s) { p.sA[s] = sg[0].A; p.sB[s] = sg[0].B; p.kp[s + 1] = p.kp[s]; }
   flops += 2LL * op.M * op.N * ktot * op.batch;
+  bytes += 8.0 * op.batch * ((double)(op.M + op.N) * ktot + (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N);
   p.M = op.M; p.N = op.N; p.K = p.kp[p.nseg];
   p.tiles_m = (op.M + BM - 1) / BM;
   p.tiles_n = (op.N + BN - 1) / BN;

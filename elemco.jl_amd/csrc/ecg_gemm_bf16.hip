@@ -12,7 +12,7 @@
 // 2. gemm_bf16x3_kernel: 256x128 output tile per 512-thread workgroup (8 waves
 //    as 4x2, 64x64 per wave = 4x4 tiles of v_mfma_f32_16x16x32_bf16), K slabs of
 //    32 double-buffered in LDS (16-byte chunks, XOR-swizzled so a 16-lane row
-//    read touches every bank once), register prefetch of the next slab, FP64
+//    read touches every bank once), register prefetch two slabs ahead, FP64
 //    split-stride epilogue or split-K partial slabs.
 // MFMA operand layout (16x16x32 BF16, gfx950): A[i][k] in lane i + 16*(k/8),
 // element k%8; B[k][j] in lane j + 16*(k/8), element k%8; D[i][j] in lane
@@ -120,36 +120,39 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   const int kend = min(p.kpad, kbeg + p.kchunk);
   const size_t kp = (size_t)p.kpad;
 
-  // global chunks: A 2 planes x 256 rows x 4 chunks (4 per thread), B 2 x 128 x 4 (2 per thread)
-  u32x4_t ra[4], rb[2];
-  auto gload = [&](int k0) {
+  // global chunks: A 2 planes x 256 rows x 4 chunks (4 per thread), B 2 x 128 x 4 (2 per thread).
+  // Two register sets: the loads of slab s+2 are issued while slab s is multiplied, so every
+  // load has two slabs of MFMA work to arrive (one slab, ~0.6 us, is shorter than the latency).
+  struct Regs { u32x4_t a[4], b[2]; };
+  Regs r0, r1;
+  auto gload = [&](Regs& r, int k0) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int idx = t + 512 * j;
       const int pl = idx >> 10, row = (idx >> 2) & (XM - 1), ch = idx & 3;
       const u16* base = pl ? p.Al : p.Ah;
-      ra[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bm + row) * kp + k0 + 8 * ch);
+      r.a[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bm + row) * kp + k0 + 8 * ch);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int idx = t + 512 * j;
       const int pl = idx >> 9, row = (idx >> 2) & (XN - 1), ch = idx & 3;
       const u16* base = pl ? p.Bl : p.Bh;
-      rb[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bn + row) * kp + k0 + 8 * ch);
+      r.b[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bn + row) * kp + k0 + 8 * ch);
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](const Regs& r, int buf) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int idx = t + 512 * j;
       const int pl = idx >> 10, row = (idx >> 2) & (XM - 1), ch = idx & 3;
-      As[buf][pl][row][swz(row, ch)] = ra[j];
+      As[buf][pl][row][swz(row, ch)] = r.a[j];
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int idx = t + 512 * j;
       const int pl = idx >> 9, row = (idx >> 2) & (XN - 1), ch = idx & 3;
-      Bs[buf][pl][row][swz(row, ch)] = rb[j];
+      Bs[buf][pl][row][swz(row, ch)] = r.b[j];
     }
   };
 
@@ -160,30 +163,19 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
   const int li = lane & 15, lk = lane >> 4;
-  int cur = 0;
-  if (kbeg < kend) {
-    gload(kbeg);
-    sstore(0);
-  }
-  __syncthreads();
-  for (int k0 = kbeg; k0 < kend; k0 += XK) {
-    const bool more = k0 + XK < kend;
-    // unconditional prefetch (the last slab reloads itself): a conditional load would make
-    // the compiler wait for it before the MFMAs below instead of before the LDS store
-    gload(more ? k0 + XK : k0);
-    asm volatile("" ::: "memory");   // keep the prefetch loads issued ahead of the MFMAs
+  auto mfma_slab = [&](int buf) {
     bf16x8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = wm * 64 + i * 16 + li;
-      ah[i] = __builtin_bit_cast(bf16x8_t, As[cur][0][r][swz(r, lk)]);
-      al[i] = __builtin_bit_cast(bf16x8_t, As[cur][1][r][swz(r, lk)]);
+      ah[i] = __builtin_bit_cast(bf16x8_t, As[buf][0][r][swz(r, lk)]);
+      al[i] = __builtin_bit_cast(bf16x8_t, As[buf][1][r][swz(r, lk)]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int c = wn * 64 + j * 16 + li;
-      bh[j] = __builtin_bit_cast(bf16x8_t, Bs[cur][0][c][swz(c, lk)]);
-      bl[j] = __builtin_bit_cast(bf16x8_t, Bs[cur][1][c][swz(c, lk)]);
+      bh[j] = __builtin_bit_cast(bf16x8_t, Bs[buf][0][c][swz(c, lk)]);
+      bl[j] = __builtin_bit_cast(bf16x8_t, Bs[buf][1][c][swz(c, lk)]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -193,9 +185,29 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
-    if (more) sstore(cur ^ 1);
+  };
+  // slab k0 of this split, clamped to the first one past the end (such loads are never stored)
+  auto slab = [&](int k0) { return k0 < kend ? k0 : kbeg; };
+  if (kbeg < kend) {
+    gload(r0, kbeg);
+    sstore(r0, 0);
+    gload(r1, slab(kbeg + XK));
+  }
+  __syncthreads();
+  // slab s sits in LDS buffer s%2; register set (s+1)%2 holds slab s+1 in flight
+  for (int k0 = kbeg; k0 < kend; k0 += 2 * XK) {
+    gload(r0, slab(k0 + 2 * XK));
+    asm volatile("" ::: "memory");   // keep the prefetch loads issued ahead of the MFMAs
+    mfma_slab(0);
+    const bool more1 = k0 + XK < kend;
+    if (more1) sstore(r1, 1);
     __syncthreads();
-    cur ^= 1;
+    if (!more1) break;
+    gload(r1, slab(k0 + 3 * XK));
+    asm volatile("" ::: "memory");
+    mfma_slab(1);
+    if (k0 + 2 * XK < kend) sstore(r0, 0);
+    __syncthreads();
   }
 
   // epilogue (FP64): D[i][j] of tile (i,j) sits in lane j + 16*(row/4), slot row%4
@@ -247,6 +259,7 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
     pa.kp[s + 1] = pb.kp[s + 1] = kpad;
   }
   flops += 2LL * op.M * op.N * ktot;
+  bytes += 8.0 * ((double)(op.M + op.N) * ktot + (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N);
   const int tiles_m = (op.M + XM - 1) / XM, tiles_n = (op.N + XN - 1) / XN;
   const long Mp = (long)tiles_m * XM, Np = (long)tiles_n * XN;
   pk.ensure(sizeof(u16) * 2 * (size_t)(Mp + Np) * kpad);

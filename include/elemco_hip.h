@@ -82,12 +82,14 @@ int ecg_cc_state_get(ecg_ctx* ctx, double* T1, double* T2);
 /* calc_cc for the closed-shell path (src/cc/cc.jl:2451-2516): iterate from the
  * MP2 guess until converged.  energies = [E_H, ESS, EOS, E_proj, dE, NormR] of
  * the last iteration; T1/T2 receive the converged amplitudes when not NULL.
- * precision = ECG_FP32 (mixed, defect correction): iterations use the residual
- * R_low(T) + delta with FP32 products; delta = R_64 - R_low is refreshed by an
- * FP64 residual at the current T every opts->refine_every-th iteration (when
- * > 0) and whenever the corrected NormR drops below 100*thr.  Convergence (the
- * reference's test) is only declared on such an exact iteration, so the result
- * is the FP64 solution; ecg_stats reports how many FP64 residuals were used. */
+ * precision = ECG_FP32 or ECG_BF16X3 (mixed, defect correction): iterations
+ * use the residual R_low(T) + delta with low-precision products; delta =
+ * R_64 - R_low is refreshed by an FP64 residual at the current T every
+ * opts->refine_every-th iteration (when > 0), whenever the corrected NormR
+ * drops below 100*thr, and whenever it stalls (less than 20% below its
+ * smallest value since the last refresh).  Convergence (the reference's test)
+ * is only declared on such an exact iteration, so the result is the FP64
+ * solution; ecg_stats reports how many FP64 residuals were used. */
 int ecg_cc_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, double* T1,
                  double* T2, double* energies, int* iterations, int* converged);
 
@@ -135,7 +137,9 @@ int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* 
  * print_time, src/tools/utils.jl:51-57): enable = 1 records HIP events around
  * every MFMA GEMM launch on the library stream and zeroes the counters.
  * stats = [last residual ms, last ladder ms, timed GEMM ms, timed GEMM launches,
- *          issued GEMM flops, FP64 iterations of the last ecg_cc_solve] (6 doubles) */
+ *          issued GEMM flops, FP64 iterations of the last ecg_cc_solve,
+ *          algorithmic GEMM bytes (operands once, C written once or read and written)]
+ * (7 doubles) */
 int ecg_profile(ecg_ctx* ctx, int enable);
 int ecg_stats(ecg_ctx* ctx, double* stats);
 
