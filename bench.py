@@ -54,6 +54,21 @@ def cpu_baseline(sample=(16, 144, 480, 7)):
                       f"({dt:.1f} s); rate uses the same algorithmic flop count as the GPU figure"}
 
 
+def measured_traffic(config):
+    """HBM bytes per FP64 GEMM launch of the o=40 v=360 iteration, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated), or None."""
+    if config != "c2":
+        return None
+    path = os.path.join(ROOT, "profiles", "ccsd_iteration_o40_v360_naux1200_fp64_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_gemm_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def time_ccsd(E, ctx, meth, prec, steps, warmup):
     opts = E.default_opts()
     ctx.state_init(opts=opts)
@@ -191,8 +206,9 @@ def main():
                    "algorithmic_tflop_per_step": round(flops / 1e12, 3)},
         "roofline": {"bound": "mfma", "kernel": "ecg::gemm_kernel<double,...> (all FP64 MFMA GEMM launches)",
                      "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                     "frac": round(achieved / PEAK["fp64"], 4), "traffic": measured_traffic(args.config),
                      "launches_per_step": launches, "avg_launch_ms": round(gemm_avg_ms, 4),
+                     "algorithmic_bytes_per_launch": round(st["gemm_bytes"] / max(1, st["gemm_launches"])),
                      "gemm_ms_per_step": round(st["gemm_ms"] / args.steps, 2),
                      "issued_tflop_per_step": round(st["gemm_flops"] / args.steps / 1e12, 3)},
     }
