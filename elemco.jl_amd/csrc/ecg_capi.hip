@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ecg_cc.h"
+#include "ecg_grad.h"
 #include "ecg_lambda.h"
 #include "ecg_triples.h"
 
@@ -272,6 +273,15 @@ int ecg_lambda_solve(ecg_ctx* ctx, int method, const ecg_solver_opts* opts, cons
     ecg::lambda_solve(c, method, to_opts(opts), T1, T2, U1, U2, &it, &conv);
     if (iterations) *iterations = it;
     if (converged) *converged = conv;
+  });
+}
+
+int ecg_energy_grad(ecg_ctx* ctx, int method, const double* T1, const double* T2, const double* U1, const double* U2,
+                    double* dE_dh, double* dE_dB) {
+  return guarded(ctx, [&](ecg::Context& c) {
+    check_method(method);
+    ECG_CHECK(T1 && T2 && U1 && U2, "NULL pointer");
+    ecg::energy_grad(c, method, T1, T2, U1, U2, dE_dh, dE_dB);
   });
 }
 

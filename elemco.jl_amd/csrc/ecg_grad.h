@@ -1,0 +1,10 @@
+// Gradient of the closed-shell CC correlation energy with respect to h and the DF factors.
+#pragma once
+#include "ecg_cc.h"
+
+namespace ecg {
+// dEdh: n x n, dEdB: n x n x naux (column-major, either may be null); T, U host arrays
+// (converged amplitudes and Lambda multipliers).
+void energy_grad(Context& c, int method, const double* T1h, const double* T2h, const double* U1h, const double* U2h,
+                 double* dEdh, double* dEdB);
+}  // namespace ecg

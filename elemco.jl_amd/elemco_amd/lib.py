@@ -55,6 +55,7 @@ _SIGS = {
     "ecg_lambda_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
     "ecg_one_rdm": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_energy_grad": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_profile": [_vp, ctypes.c_int],
     "ecg_stats": [_vp, _dp],
 }
@@ -243,6 +244,16 @@ class Context:
         D1, dD1 = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
         self._chk(self._lib.ecg_one_rdm(self._h, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2), _ptr(D1), _ptr(dD1)))
         return D1, dD1
+
+    def energy_grad(self, T1, T2, U1, U2, method=ECG_CCSD):
+        """``ecg_energy_grad``: (dE/dh [n, n], dE/dB [n, n, naux]) at converged T and U."""
+        T1, T2, U1, U2 = self._t1(T1), self._t2(T2), self._t1(U1), self._t2(U2)
+        n = self.norb
+        gh = np.zeros((n, n), order="F")
+        gb = np.zeros((n, n, self.naux), order="F")
+        self._chk(self._lib.ecg_energy_grad(self._h, method, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2), _ptr(gh),
+                                            _ptr(gb)))
+        return gh, gb
 
     def profile(self, enable=True):
         """Record HIP events around every MFMA GEMM launch (``ecg_profile``)."""

@@ -129,6 +129,17 @@ int ecg_lambda_residual(ecg_ctx* ctx, int method, int precision, const double* T
 int ecg_lambda_solve(ecg_ctx* ctx, int method, const ecg_solver_opts* opts, const double* T1,
                      const double* T2, double* U1, double* U2, int* iterations, int* converged);
 
+/* Gradient of the correlation energy (the backward of the CC energy functional;
+ * the reference has no counterpart -- its Lambda equations, lm_cc_iterations!
+ * src/cc/cc_lagrange.jl:1155-1223, and calc_1RDM :793-837 are the pieces it
+ * builds on).  At converged amplitudes T1, T2 (ecg_cc_solve) and multipliers U1,
+ * U2 (ecg_lambda_solve) of the same method: dE_dh[p,q] = dE/dh[p,q] (n x n) and
+ * dE_dB[p,q,Q] = dE/dB[p,q,Q] (n x n x naux), every entry of h and B treated as
+ * independent; either output may be NULL.  dE/dF at fixed B equals dE/dh.
+ * Hermitian integrals (BR = NULL at ecg_create) only. */
+int ecg_energy_grad(ecg_ctx* ctx, int method, const double* T1, const double* T2, const double* U1,
+                    const double* U2, double* dE_dh, double* dE_dB);
+
 /* One-particle density: calc_1RDM (src/cc/cc_lagrange.jl:793-837); D1, dD1 n x n. */
 int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* U1, const double* U2,
                 double* D1, double* dD1);
