@@ -156,12 +156,89 @@ def time_triples(E, ctx, prec, nsample, o, v):
             "full_T_estimate_s": round(dt / nsample * triplet_count(o), 1)}
 
 
+def _solve_from(E, o, h, B, T1, T2, meth, opts):
+    """FP64 solve of the system (h, B) started from amplitudes T1, T2; returns (E_corr, iterations, converged)."""
+    ctx = E.Context(o, h.shape[0] - o, B.shape[2], h, B)
+    try:
+        ctx.state_init(T1, T2, opts)
+        for it in range(1, opts.maxit + 1):
+            info = ctx.state_step(meth, E.ECG_FP64, opts)
+            if info[7]:
+                break
+    finally:
+        ctx.close()
+    return float(info[0]), it, bool(info[7])
+
+
+def gradient_check(E, o, v, q, seed, meth, ndir=20, step=1e-4, thr=1e-20):
+    """dE/dF (Fock diagonal at fixed B) and dE/dB from ecg_energy_grad against central finite
+    differences (step h) of fully re-solved energies, along ndir seeded random directions: the
+    first half perturb the Fock diagonal, the second half B (symmetric in p, q, scaled to the
+    width of B).  Also compares dE/dh with the transposed Lambda one-particle density dD1.
+    Runs on the conditioned system of the given sizes; perturbed solves start from the
+    converged amplitudes of the unperturbed one."""
+    import numpy as np
+    from elemco_amd.synthetic import conditioned_scale, make_system
+    sc = conditioned_scale(o, v, q)
+    _, h, B, _ = make_system(o, v, q, seed, sc)
+    opts = E.default_opts(thr=thr, maxit=200)
+    ctx = E.Context(o, v, q, h, B)
+    out = {"system": f"o={o} v={v} naux={q} seed={seed}, factor width 0.05*{sc:.4f}", "thr": thr, "step": step,
+           "method": "dcsd" if meth == E.ECG_DCSD else "ccsd"}
+    t0 = time.perf_counter()
+    e, T1, T2, it, conv = ctx.cc_solve(meth, E.ECG_FP64, opts)
+    out["cc_solve"] = {"E_corr": e[0], "iterations": it, "converged": conv,
+                       "seconds": round(time.perf_counter() - t0, 2)}
+    log(f"  cc solve: E={e[0]:.14f} it={it} conv={conv}")
+    t0 = time.perf_counter()
+    U1, U2, lit, lconv = ctx.lambda_solve(T1, T2, meth, opts)
+    out["lambda_solve"] = {"iterations": lit, "converged": lconv, "seconds": round(time.perf_counter() - t0, 2)}
+    log(f"  lambda solve: it={lit} conv={lconv}")
+    ctx.energy_grad(T1, T2, U1, U2, meth)                       # warm-up (allocations)
+    t0 = time.perf_counter()
+    gh, gb = ctx.energy_grad(T1, T2, U1, U2, meth)
+    out["grad_seconds"] = round(time.perf_counter() - t0, 3)
+    t0 = time.perf_counter()
+    ctx.cc_residual(T1, T2, meth)
+    out["residual_seconds"] = round(time.perf_counter() - t0, 3)
+    _, dD1 = ctx.one_rdm(T1, T2, U1, U2)
+    out["max_abs_dEdh_minus_dD1T"] = float(np.max(np.abs(gh - dD1.T)))
+    ctx.close()
+    rng = np.random.default_rng(seed)
+    width = float(np.std(B))
+    dirs = []
+    for k in range(ndir):
+        if k < ndir // 2:
+            dh = np.diag(rng.standard_normal(h.shape[0]))
+            dB = np.zeros_like(B)
+            an = float(np.sum(gh * dh))
+            kind = "F_pp"
+        else:
+            dh = np.zeros_like(h)
+            dB = width * rng.standard_normal(B.shape)
+            dB = 0.5 * (dB + dB.transpose(1, 0, 2))
+            an = float(np.sum(gb * dB))
+            kind = "B^Q"
+        ep, itp, cp = _solve_from(E, o, h + step * dh, B + step * dB, T1, T2, meth, opts)
+        em, itm, cm = _solve_from(E, o, h - step * dh, B - step * dB, T1, T2, meth, opts)
+        fd = (ep - em) / (2.0 * step)
+        rel = abs(fd - an) / abs(an)
+        dirs.append({"kind": kind, "analytic": an, "finite_difference": fd, "rel_err": rel,
+                     "iterations": [itp, itm], "converged": cp and cm})
+        log(f"  direction {k} ({kind}): analytic {an:.12e} fd {fd:.12e} rel {rel:.2e} it {itp},{itm}")
+    out["directions"] = dirs
+    out["max_rel_err"] = max(d["rel_err"] for d in dirs)
+    out["all_converged"] = all(d["converged"] for d in dirs) and conv and lconv
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default=None, choices=sorted(CONFIGS),
+                    help="workload sizes (default c2; c3 with --gradcheck)")
     ap.add_argument("--method", default="ccsd", choices=["ccsd", "dcsd"])
     ap.add_argument("--triplets", type=int, default=48, help="(T) sample size; 0 skips (T)")
     ap.add_argument("--no-mixed", action="store_true", help="time only the FP64 iteration")
@@ -170,7 +247,12 @@ def main():
                     help="only the GEMM-engine rates on 8192^3 and a 1 GiB copy (one JSON object)")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the full FP64 / mixed solves on the conditioned system of the same sizes")
+    ap.add_argument("--gradcheck", action="store_true",
+                    help="only the energy-gradient check against finite differences (one JSON object)")
+    ap.add_argument("--directions", type=int, default=20, help="random directions of the gradient check")
     args = ap.parse_args()
+    if args.config is None:
+        args.config = "c3" if args.gradcheck else "c2"
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
@@ -184,11 +266,15 @@ def main():
     from elemco_amd.flops import ccsd_iteration
 
     o, v, q, seed = CONFIGS[args.config]
+    meth = E.ECG_DCSD if args.method == "dcsd" else E.ECG_CCSD
+    if args.gradcheck:
+        log(f"gradient check o={o} v={v} naux={q}")
+        print(json.dumps(gradient_check(E, o, v, q, seed, meth, args.directions)), flush=True)
+        return
     t0 = time.time()
     EC = E.ECInfo.synthetic(o, v, q, seed)
     log(f"setup o={o} v={v} naux={q}: {time.time() - t0:.1f} s")
     ctx = EC.ctx
-    meth = E.ECG_DCSD if args.method == "dcsd" else E.ECG_CCSD
     flops = ccsd_iteration(o, v, q, dcsd=(args.method == "dcsd"))
 
     log("FP64 iterations")

@@ -183,4 +183,20 @@ function calc_lm_cc(ctx::ECGContext, T1, T2, method::AbstractString; opts=defaul
   return U1, U2
 end
 
+"""
+`energy_grad(ctx, T1, T2, U1, U2, method)`: `(dE/dh [n,n], dE/dB [n,n,naux])` of the
+correlation energy at converged amplitudes and Λ multipliers (no reference
+counterpart; the backward a ChainRulesCore `rrule` of the CC energy would call).
+"""
+function energy_grad(ctx::ECGContext, T1, T2, U1, U2, method::AbstractString)
+  m = lowercase(method) == "dcsd" ? ECG_DCSD : ECG_CCSD
+  n = ctx.nocc + ctx.nvirt
+  gh, gb = zeros(n, n), zeros(n, n, ctx.naux)
+  st = ccall((:ecg_energy_grad, LIB), Cint,
+             (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+             ctx.handle, m, T1, T2, U1, U2, gh, gb)
+  _check(ctx, st)
+  return gh, gb
+end
+
 end # module
