@@ -250,6 +250,8 @@ def main():
     ap.add_argument("--gradcheck", action="store_true",
                     help="only the energy-gradient check against finite differences (one JSON object)")
     ap.add_argument("--directions", type=int, default=20, help="random directions of the gradient check")
+    ap.add_argument("--triples-only", action="store_true",
+                    help="only the (T) rates per precision on --triplets sampled triplets (one JSON object)")
     args = ap.parse_args()
     if args.config is None:
         args.config = "c3" if args.gradcheck else "c2"
@@ -275,6 +277,15 @@ def main():
     EC = E.ECInfo.synthetic(o, v, q, seed)
     log(f"setup o={o} v={v} naux={q}: {time.time() - t0:.1f} s")
     ctx = EC.ctx
+    if args.triples_only:
+        out = {"workload": f"(T) on MP2 amplitudes, o={o} v={v} naux={q} seed={seed}"}
+        for name, prec in (("fp64", E.ECG_FP64), ("fp32", E.ECG_FP32), ("bf16x3", E.ECG_BF16X3)):
+            log(f"(T) {name}")
+            out[name] = time_triples(E, ctx, prec, args.triplets, o, v)
+        out["bf16x3"]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_triplet"] / out["bf16x3"]["ms_per_triplet"], 3)
+        out["fp32"]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_triplet"] / out["fp32"]["ms_per_triplet"], 3)
+        print(json.dumps(out), flush=True)
+        return
     flops = ccsd_iteration(o, v, q, dcsd=(args.method == "dcsd"))
 
     log("FP64 iterations")

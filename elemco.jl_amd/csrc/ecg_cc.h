@@ -94,6 +94,7 @@ struct Context {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double last_residual_ms = 0.0, last_ladder_ms = 0.0;
   int fp64_iterations = 0;   // FP64 iterations of the last solve (mixed precision)
+  DevBuf tws[25];            // (T) work buffers: kept between calls, grown on demand
   ~Context();
 };
 

@@ -54,6 +54,33 @@ struct GemmOp {
   }
 };
 
+// A product on operands already packed as BF16 hi/lo planes (pack_rows_bf16x3):
+// segment s reads A rows m at Ah/Al + m*lda and B rows n at Bh/Bl + n*ldb, each
+// row K-contiguous and zero-padded to a multiple of 32 (K is that padded
+// length).  Rows up to the next multiple of 256 (A) and 128 (B) past M and N are
+// read, so the planes need that much readable memory after the last row (the
+// results of those rows are discarded).  C is FP64 or FP32, split-stride
+// addressed as in GemmOp.
+struct PackedSeg {
+  const uint16_t* Ah = nullptr; const uint16_t* Al = nullptr; long lda = 0;
+  const uint16_t* Bh = nullptr; const uint16_t* Bl = nullptr; long ldb = 0;
+  int K = 0;
+};
+struct PackedOp {
+  int M = 0, N = 0, nseg = 0;
+  PackedSeg seg[MAXSEG];
+  double* C = nullptr; float* Cf = nullptr;   // exactly one is set
+  long c_ms0 = 1, c_ms1 = 0, c_ns0 = 0, c_ns1 = 0;
+  int c_md0 = 1 << 30, c_nd0 = 1 << 30;
+  double alpha = 1.0, beta = 0.0;
+  long long algorithmic_k = 0;                 // unpadded K summed over segments (flop count)
+  // box_class = 1, 2, 3: instead of the split strides, C is a tensor K[x1,x2,x3] (each
+  // index < box_v) stored in 8x8x8 boxes: element at ((x1/8) + nt (x2/8 + nt x3/8)) * 512
+  // + x1%8 + 8 (x2%8) + 64 (x3%8), nt = ceil(box_v / 8).  With m = y1 + box_v y2 the
+  // coordinates are (n, y1, y2) for class 1, (y1, n, y2) for 2 and (y1, y2, n) for 3.
+  int box_class = 0, box_v = 0;
+};
+
 // Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,
 // FP32 accumulation), or BF16 hi/lo split (three BF16 MFMA products per pair,
 // FP32 accumulation).  Results are always returned in FP64.
@@ -76,6 +103,12 @@ struct GemmEngine {
   // K-contiguous BF16 hi and lo planes in `pk`, 256x128 MFMA tiles
   DevBuf pk;
   void run_bf16x3(const GemmOp& op);
+  // Pack rows r < rows of src (row offset split as in GemmOp: (r % d0)*s0 + (r / d0)*s1,
+  // element k at + k*sk, k < K) into hi/lo planes [r][kpad], kpad a multiple of 32 >= K.
+  // Rows are written up to the next multiple of 64.
+  void pack_rows_bf16x3(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad,
+                        uint16_t* hi, uint16_t* lo);
+  void run_bf16x3_packed(const PackedOp& op);
   void reduce_slabs(const GemmOp& op, int ksplit);   // split-K partial slabs in ws -> C
   hipEvent_t timing_begin();
   void timing_end(hipEvent_t e0);

@@ -97,15 +97,31 @@ __global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p, int nrt) {
 
 struct X3Params {
   int M, N, kpad, kchunk, ksplit, tiles_m, tiles_n;
-  const u16* Ah; const u16* Al;   // [tiles_m*256][kpad]
-  const u16* Bh; const u16* Bl;   // [tiles_n*128][kpad]
-  double* C; long c_ms0, c_ms1, c_ns0, c_ns1, c_z; int c_md0, c_nd0;
+  // K segments: slab k0 in [kb[s], kb[s+1]) reads row m of A at Ah[s] + m*lda[s] + (k0 - kb[s])
+  int nseg, kb[MAXSEG + 1];
+  const u16* Ah[MAXSEG]; const u16* Al[MAXSEG]; long lda[MAXSEG];
+  const u16* Bh[MAXSEG]; const u16* Bl[MAXSEG]; long ldb[MAXSEG];
+  double* C; float* Cf; long c_ms0, c_ms1, c_ns0, c_ns1, c_z; int c_md0, c_nd0;
   double alpha, beta;
   int cplain; long ldc_ws;
+  int box_class, box_v;   // boxed output (see PackedOp)
 };
+
+// Offset of element (m, n) of a boxed output (PackedOp::box_class).
+__device__ __forceinline__ long box_off(int m, int n, int cls, int bv) {
+  const int y2 = m / bv, y1 = m - y2 * bv;
+  const int x1 = cls == 1 ? n : y1, x2 = cls == 1 ? y1 : (cls == 2 ? n : y2), x3 = cls == 3 ? n : y2;
+  const long nt = (bv + 7) >> 3;
+  return (((long)(x1 >> 3) + nt * ((x2 >> 3) + nt * (long)(x3 >> 3))) << 9) + (x1 & 7) + ((x2 & 7) << 3) +
+         ((x3 & 7) << 6);
+}
 
 __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 3); }
 
+// TO (transposed output): the MFMAs take B as their first operand, so each
+// accumulator holds D^T -- lanes run along m and the four slots along n -- and
+// the epilogue stores stay contiguous when m is the unit-stride output index.
+template <bool TO>
 __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   __shared__ u32x4_t As[2][2][XM][4];   // [buffer][plane][row][16-byte chunk]
   __shared__ u32x4_t Bs[2][2][XN][4];
@@ -118,7 +134,6 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   const int kz = blockIdx.z;
   const int kbeg = kz * p.kchunk;
   const int kend = min(p.kpad, kbeg + p.kchunk);
-  const size_t kp = (size_t)p.kpad;
 
   // global chunks: A 2 planes x 256 rows x 4 chunks (4 per thread), B 2 x 128 x 4 (2 per thread).
   // Two register sets: the loads of slab s+2 are issued while slab s is multiplied, so every
@@ -126,19 +141,29 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   struct Regs { u32x4_t a[4], b[2]; };
   Regs r0, r1;
   auto gload = [&](Regs& r, int k0) {
+    // segment of this slab (uniform; static indices only, so the arguments stay in SGPRs)
+    const u16* ah = p.Ah[0]; const u16* al = p.Al[0]; const u16* bh = p.Bh[0]; const u16* bl = p.Bl[0];
+    long lda = p.lda[0], ldb = p.ldb[0];
+    int kl = k0;
+#pragma unroll
+    for (int q = 1; q < MAXSEG; ++q)
+      if (q < p.nseg && k0 >= p.kb[q]) {
+        ah = p.Ah[q]; al = p.Al[q]; bh = p.Bh[q]; bl = p.Bl[q];
+        lda = p.lda[q]; ldb = p.ldb[q]; kl = k0 - p.kb[q];
+      }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int idx = t + 512 * j;
       const int pl = idx >> 10, row = (idx >> 2) & (XM - 1), ch = idx & 3;
-      const u16* base = pl ? p.Al : p.Ah;
-      r.a[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bm + row) * kp + k0 + 8 * ch);
+      const u16* base = pl ? al : ah;
+      r.a[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bm + row) * lda + kl + 8 * ch);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int idx = t + 512 * j;
       const int pl = idx >> 9, row = (idx >> 2) & (XN - 1), ch = idx & 3;
-      const u16* base = pl ? p.Bl : p.Bh;
-      r.b[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bn + row) * kp + k0 + 8 * ch);
+      const u16* base = pl ? bl : bh;
+      r.b[j] = *reinterpret_cast<const u32x4_t*>(base + (size_t)(bn + row) * ldb + kl + 8 * ch);
     }
   };
   auto sstore = [&](const Regs& r, int buf) {
@@ -181,9 +206,15 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        if constexpr (TO) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+        } else {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
       }
   };
   // slab k0 of this split, clamped to the first one past the end (such loads are never stored)
@@ -210,24 +241,56 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
     __syncthreads();
   }
 
-  // epilogue (FP64): D[i][j] of tile (i,j) sits in lane j + 16*(row/4), slot row%4
+  // epilogue: element (row, col) of tile (i,j) sits in lane col + 16*(row/4), slot row%4
+  // (TO: lane row + 16*(col/4), slot col%4).  Per column group: all 16 offsets, then
+  // all loads of C, then all stores (a load issued after a store to C could alias it,
+  // which would serialise every element).
   double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int n = bn + wn * 64 + j * 16 + li;
-    if (n >= p.N) continue;
-    const long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
+    long off[16];
+    bool ok[16];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = bm + wm * 64 + i * 16 + 4 * lk + r;
-        if (m >= p.M) continue;
-        const long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
-        double v = p.alpha * (double)acc[i][j][r];
-        if (p.beta != 0.0) v += p.beta * C[cm + cn];
-        C[cm + cn] = v;
+        const int n = bn + wn * 64 + j * 16 + (TO ? 4 * lk + r : li);
+        const int m = bm + wm * 64 + i * 16 + (TO ? li : 4 * lk + r);
+        const int e = 4 * i + r;
+        ok[e] = n < p.N && m < p.M;
+        long oe;
+        if (p.box_class) {
+          oe = box_off(m, n, p.box_class, p.box_v);
+        } else {
+          const long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
+          const long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
+          oe = cm + cn;
+        }
+        off[e] = ok[e] ? oe : 0;
       }
+    const bool rd = p.beta != 0.0;
+    if (p.Cf) {
+      float cv[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? p.Cf[off[e]] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = 4 * i + r;
+          if (ok[e]) p.Cf[off[e]] = (float)(p.alpha * (double)acc[i][j][r] + (rd ? p.beta * (double)cv[e] : 0.0));
+        }
+    } else {
+      double cv[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? C[off[e]] : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = 4 * i + r;
+          if (ok[e]) C[off[e]] = p.alpha * (double)acc[i][j][r] + (rd ? p.beta * cv[e] : 0.0);
+        }
     }
   }
 }
@@ -277,7 +340,11 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
 
   X3Params x{};
   x.M = op.M; x.N = op.N; x.kpad = kpad; x.tiles_m = tiles_m; x.tiles_n = tiles_n;
-  x.Ah = Ah; x.Al = Al; x.Bh = Bh; x.Bl = Bl;
+  x.nseg = 1; x.kb[0] = 0;
+  for (int s = 1; s <= MAXSEG; ++s) x.kb[s] = kpad;
+  for (int s = 0; s < MAXSEG; ++s) {
+    x.Ah[s] = Ah; x.Al[s] = Al; x.Bh[s] = Bh; x.Bl[s] = Bl; x.lda[s] = kpad; x.ldb[s] = kpad;
+  }
   x.alpha = op.alpha; x.beta = op.beta;
   // split K when the tiles alone cannot occupy the 256 CUs (one workgroup per CU)
   const long tiles = (long)tiles_m * tiles_n;
@@ -302,10 +369,74 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   }
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
-  hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0, stream, x);
+  if (x.cplain || (x.c_ms0 == 1 && x.c_ns0 != 1))
+    hipLaunchKernelGGL(gemm_bf16x3_kernel<true>, dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0, stream, x);
+  else
+    hipLaunchKernelGGL(gemm_bf16x3_kernel<false>, dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0, stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
   if (ksplit > 1) reduce_slabs(op, ksplit);
+}
+
+void GemmEngine::pack_rows_bf16x3(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad,
+                                  uint16_t* hi, uint16_t* lo) {
+  ECG_CHECK(K > 0 && kpad % XK == 0 && kpad >= K, "pack: kpad must be a multiple of 32 covering K");
+  const long rt = (rows + PR - 1) / PR;
+  ECG_CHECK(rt * PR < (1L << 31) && rt * (kpad / XK) < (1L << 31), "pack: operand too large for the packing grid");
+  PackArgs pa{};
+  pa.rows = (int)rows; pa.kpad = kpad; pa.nseg = 1;
+  for (int s = 0; s < MAXSEG; ++s) { pa.src[s] = src; pa.sk[s] = sk; pa.sK[s] = K; }
+  pa.kp[0] = 0;
+  for (int s = 1; s <= MAXSEG; ++s) pa.kp[s] = kpad;
+  pa.s0 = s0; pa.s1 = s1; pa.d0 = d0; pa.hi = hi; pa.lo = lo;
+  hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)(rt * (kpad / XK))), dim3(256), 0, stream, pa, (int)rt);
+  ECG_HIP(hipGetLastError());
+}
+
+void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
+  ECG_CHECK(op.nseg >= 1 && op.nseg <= MAXSEG, "gemm: packed product needs 1..4 K segments");
+  ECG_CHECK((op.C != nullptr) != (op.Cf != nullptr), "gemm: packed product needs exactly one output");
+  X3Params x{};
+  x.M = op.M; x.N = op.N;
+  x.tiles_m = (op.M + XM - 1) / XM; x.tiles_n = (op.N + XN - 1) / XN;
+  x.nseg = op.nseg;
+  int kpad = 0;
+  for (int s = 0; s < op.nseg; ++s) {
+    const PackedSeg& g = op.seg[s];
+    ECG_CHECK(g.K > 0 && g.K % XK == 0 && g.Ah && g.Al && g.Bh && g.Bl && g.lda >= g.K && g.ldb >= g.K,
+              "gemm: packed segment needs K a positive multiple of 32 and row strides >= K");
+    x.kb[s] = kpad;
+    x.Ah[s] = g.Ah; x.Al[s] = g.Al; x.lda[s] = g.lda;
+    x.Bh[s] = g.Bh; x.Bl[s] = g.Bl; x.ldb[s] = g.ldb;
+    kpad += g.K;
+  }
+  for (int s = op.nseg; s < MAXSEG; ++s) {
+    x.kb[s] = kpad;
+    x.Ah[s] = x.Ah[0]; x.Al[s] = x.Al[0]; x.lda[s] = x.lda[0];
+    x.Bh[s] = x.Bh[0]; x.Bl[s] = x.Bl[0]; x.ldb[s] = x.ldb[0];
+  }
+  x.kb[MAXSEG] = kpad;
+  x.kpad = kpad; x.kchunk = kpad; x.ksplit = 1;
+  x.C = op.C; x.Cf = op.Cf; x.cplain = 0;
+  x.c_ms0 = op.c_ms0; x.c_ms1 = op.c_ms1; x.c_md0 = op.c_md0;
+  x.c_ns0 = op.c_ns0; x.c_ns1 = op.c_ns1; x.c_nd0 = op.c_nd0;
+  x.alpha = op.alpha; x.beta = op.beta;
+  x.box_class = op.box_class; x.box_v = op.box_v;
+  ECG_CHECK(op.box_class == 0 || ((long)op.box_v * op.box_v == op.M && op.box_v == op.N),
+            "gemm: a boxed output needs M = box_v^2 and N = box_v");
+  flops += 2LL * op.M * op.N * op.algorithmic_k;
+  bytes += (op.Cf ? 4.0 : 8.0) * (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N +
+           4.0 * (double)(op.M + op.N) * op.algorithmic_k;
+  const long tiles = (long)x.tiles_m * x.tiles_n;
+  ECG_CHECK(tiles < (1L << 31), "gemm: too many tiles");
+  hipEvent_t e0 = timing_begin();
+  // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
+  if (op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1))
+    hipLaunchKernelGGL(gemm_bf16x3_kernel<true>, dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
+  else
+    hipLaunchKernelGGL(gemm_bf16x3_kernel<false>, dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
+  ECG_HIP(hipGetLastError());
+  timing_end(e0);
 }
 
 }  // namespace ecg

@@ -32,29 +32,48 @@ namespace {
 constexpr int TT = 8;                       // combine tile edge
 constexpr int NPART = 6 * TT + 1;           // per-block partial record: 6 vectors x 8 + energy
 
+template <typename KT>
 struct CombArgs {
   int nv, no, i, j, k, ntile;
-  const double* K;      // v^3, [a,b,c]
-  const double* KU;     // Lambda: second intermediate, else null
+  const KT* K[3];       // v^3, [a,b,c]: K is the sum of the three (one per product class)
+  const KT* KU[3];      // Lambda: the second intermediate, likewise; else null
   const double* eo;     // occupied orbital energies
   const double* ev;     // virtual orbital energies
   const double* W;      // <ij|ab> as [a,b,i,j]
   const double* Y;      // T2 (or covariant U2) [a,b,i,j] for IntY
-  const int* tri;       // sorted tile triples (A <= B <= C), 3 ints each, one per workgroup
+  const int* tri;       // sorted tile triples (A <= B <= C), 3 ints each
+  int boxed;            // K (and KU) stored in 8x8x8 boxes (GemmEngine boxed output), else [a,b,c]
   double* part;         // [ntile^3][NPART], record of output block at + nt*(bt + nt*ct)
+  // per tie pattern (A==B) + 2 (B==C): the distinct output blocks, each as its
+  // permutation index and the box slot of each of the six permutations of K
+  int nblk[4];
+  signed char blk[4][6][7];
 };
 
+// padded LDS strides of a box: (9, 76) keeps every access pattern of the combine
+// (six permuted box reads, X stores, the dot products) at most 3-way bank-conflicted
+constexpr int BX1 = TT + 1, BX2 = 76;
+
 // One workgroup per sorted tile triple (A,B,C).  The six boxes K[perm(A,B,C)]
-// (8x8x8 each) are staged in LDS once and serve every distinct output block
-// (at,bt,ct) that is a permutation of (A,B,C): all six index permutations of
-// K that the combine of such a block reads lie in those boxes, so K is read
-// once per triplet instead of six times.
-__global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
-  __shared__ double box[6][TT * TT * TT];
-  __shared__ double wsh[6][TT * TT];    // <jk|bc>-type slices (0-2) and Y slices (3-5) of the block
-  __shared__ double esh[3][TT];         // virtual orbital energies of the block's a, b, c tiles
-  __shared__ double pab[6][4][TT];      // per-wave partials of the a- and b-indexed vectors
-  __shared__ double pcc[6][TT];         // c-indexed vectors
+// (8x8x8; K summed over its three parts) are staged in LDS once and serve every
+// distinct output block (at,bt,ct) that is a permutation of (A,B,C): all six
+// index permutations of K that the combine of such a block reads lie in those
+// boxes, so K is read once per triplet.  The 8x8 slices of W and Y that the
+// output blocks multiply X with (36: two tensors, three occupied pairs, six
+// ordered pairs of the three tiles) and the tiles' orbital energies are staged
+// in the same pass.  Per output block, X is formed (two elements per thread)
+// into LDS, and the 48 vector entries of the block (IntX and IntY over a, b
+// and c) are 64-term dot products of X with a slice, four lanes per entry and
+// a two-step shuffle.  The energy sum K.X is accumulated per thread over the
+// whole triple and reduced once; it is recorded with the first output block.
+// All sums run in a fixed order (deterministic).  X is formed in the type of K
+// (FP32 for the BF16 hi/lo path); the sums with W and Y are FP64.
+template <typename KT>
+__global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
+  __shared__ KT box[6][TT * BX2];
+  __shared__ KT xs[TT * BX2];                // X of the current output block
+  __shared__ double wsl[2][3][6][TT * TT];   // [W|Y][occupied pair][ordered tile pair][x + 8 y]
+  __shared__ double esl[3][TT];              // virtual orbital energies of tiles T[0], T[1], T[2]
   __shared__ double ered[4];
   const int nt = p.ntile;
   const long v = p.nv;
@@ -63,115 +82,121 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
   const int T[3] = {p.tri[3 * blockIdx.x], p.tri[3 * blockIdx.x + 1], p.tri[3 * blockIdx.x + 2]};
   // permutation table: which of the three coordinates sits at K index 1,2,3
   const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}, {2, 0, 1}, {1, 2, 0}};
-  // slot s holds K at tile coordinates (T[perm[s][0]], T[perm[s][1]], T[perm[s][2]])
-  for (int s = 0; s < 6; ++s) {
-    const int b1 = T[perm[s][0]] * TT, b2 = T[perm[s][1]] * TT, b3 = T[perm[s][2]] * TT;
-    for (int e = t; e < TT * TT * TT; e += blockDim.x) {
-      int x1 = b1 + e % TT, x2 = b2 + (e / TT) % TT, x3 = b3 + e / (TT * TT);
-      bool ok = x1 < v && x2 < v && x3 < v;
-      box[s][e] = ok ? p.K[(long)x1 + v * x2 + v * v * x3] : 0.0;
-    }
-  }
-  const double eijk = p.eo[p.i] + p.eo[p.j] + p.eo[p.k];
+  // ordered pair (p, q), p != q, of tile positions -> 0..5
+  auto pair = [](int a, int b) { return 2 * a + (b > a ? b - 1 : b); };
+  // occupied pair of each role: 0 -> (j,k), 1 -> (i,k), 2 -> (i,j)
   const long ooff[3] = {v * v * (p.j + (long)p.no * p.k), v * v * (p.i + (long)p.no * p.k),
                         v * v * (p.i + (long)p.no * p.j)};
-  for (int o = 0; o < 6; ++o) {
-    // output block tile coordinates; skip permutations equal to an earlier one (ties)
-    const int tg[3] = {T[perm[o][0]], T[perm[o][1]], T[perm[o][2]]};
-    bool dup = false;
-    for (int o2 = 0; o2 < o; ++o2)
-      dup = dup || (T[perm[o2][0]] == tg[0] && T[perm[o2][1]] == tg[1] && T[perm[o2][2]] == tg[2]);
-    if (dup) continue;
-    const int base[3] = {tg[0] * TT, tg[1] * TT, tg[2] * TT};
-    // stage the 8x8 slices of W and Y this block multiplies X with, and the orbital energies:
-    // slice q%3 = 0 over (b,c) at (j,k), 1 over (a,c) at (i,k), 2 over (a,b) at (i,j)
-    for (int u = t; u < 6 * TT * TT; u += blockDim.x) {
-      const int q = u / (TT * TT), x = u % TT, y = (u / TT) % TT;
-      const int role = q % 3;
-      const int d1 = role == 0 ? 1 : 0, d2 = role == 2 ? 1 : 2;   // the two coordinates of the slice
-      const int c1 = base[d1] + x, c2 = base[d2] + y;
-      const double* src = q < 3 ? p.W : p.Y;
-      const bool ok = c1 < v && c2 < v;
-      wsh[q][x + TT * y] = ok ? src[c1 + v * c2 + ooff[role]] : 0.0;
+  // ---- staging (all global loads of the workgroup)
+#pragma unroll
+  for (int s = 0; s < 6; ++s)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = t + 256 * u;
+      const int x1 = T[perm[s][0]] * TT + e % TT, x2 = T[perm[s][1]] * TT + (e / TT) % TT,
+                x3 = T[perm[s][2]] * TT + e / (TT * TT);
+      const bool ok = x1 < v && x2 < v && x3 < v;
+      long off;
+      if (p.boxed)
+        off = ((long)T[perm[s][0]] + nt * ((long)T[perm[s][1]] + nt * (long)T[perm[s][2]])) * (TT * TT * TT) + e;
+      else
+        off = ok ? (long)x1 + v * x2 + v * v * x3 : 0;
+      box[s][e % TT + BX1 * ((e / TT) % TT) + BX2 * (e / (TT * TT))] =
+          ok ? p.K[0][off] + p.K[1][off] + p.K[2][off] : KT(0);
     }
-    if (t < 3 * TT) {
-      const int d = t / TT, x = base[d] + t % TT;
-      esh[d][t % TT] = x < v ? p.ev[x] : 0.0;
-    }
-    // the box holding permutation q of this block's coordinates
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const int u = t + 256 * q;
+    const int x = u % TT, y = (u / TT) % TT, pr = (u / (TT * TT)) % 6, role = (u / (6 * TT * TT)) % 3;
+    const int wy = u / (18 * TT * TT);
+    const int pa = pr / 2, pb = (pr % 2 < pa) ? pr % 2 : pr % 2 + 1;   // inverse of pair()
+    const int c1 = T[pa] * TT + x, c2 = T[pb] * TT + y;
+    const double* src = wy == 0 ? p.W : p.Y;
+    const bool ok = c1 < v && c2 < v;
+    (&wsl[0][0][0][0])[u] = ok ? src[ok ? c1 + v * c2 + ooff[role] : 0] : 0.0;
+  }
+  if (t < 3 * TT) {
+    const int x = T[t / TT] * TT + t % TT;
+    esl[t / TT][t % TT] = x < v ? p.ev[x < v ? x : 0] : 0.0;
+  }
+  const double eijk = p.eo[p.i] + p.eo[p.j] + p.eo[p.k];
+  __syncthreads();
+
+  // dot-product assignment of the 192 first threads: entry q = t/4 (vector q/8, index q%8), part t%4
+  const int dq = t >> 2, dpart = t & 3, dvec = dq >> 3, didx = dq & 7;
+  const int drole = dvec % 3, dwy = dvec / 3;
+  double esum = 0.0;
+  const int pat = (T[0] == T[1] ? 1 : 0) + (T[1] == T[2] ? 2 : 0);
+  const int nblk = p.nblk[pat];
+  for (int nb = 0; nb < nblk; ++nb) {
+    // the nb-th distinct output block: permutation o of the triple, and the box holding
+    // permutation q of its coordinates
+    const int o = p.blk[pat][nb][0];
     int slot[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int w0 = tg[perm[q][0]], w1 = tg[perm[q][1]], w2 = tg[perm[q][2]];
-      int sl = 0;
-      for (int s = 5; s >= 0; --s)
-        if (T[perm[s][0]] == w0 && T[perm[s][1]] == w1 && T[perm[s][2]] == w2) sl = s;
-      slot[q] = sl;
-    }
-    __syncthreads();   // boxes (first block) and slices staged
-    double esum = 0.0;
-    double rs[2][6];
+    for (int q = 0; q < 6; ++q) slot[q] = p.blk[pat][nb][1 + q];
+    const int tg[3] = {T[perm[o][0]], T[perm[o][1]], T[perm[o][2]]};
+    const double* ea = esl[perm[o][0]];
+    const double* eb = esl[perm[o][1]];
+    const double* ec = esl[perm[o][2]];
+    // X of the block (zero outside the virtual range)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int lc = w + 4 * h;
-      const int a = base[0] + la, b = base[1] + lb, c = base[2] + lc;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) rs[h][q] = 0.0;
+      const int a = tg[0] * TT + la, b = tg[1] * TT + lb, c = tg[2] * TT + lc;
+      KT xv = KT(0);
       if (a < v && b < v && c < v) {
-        // local coordinates (la,lb,lc) read in the order of each permutation
         const int lo[3] = {la, lb, lc};
-        double kv[6];
+        KT kv[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-          kv[q] = box[slot[q]][lo[perm[q][0]] + TT * lo[perm[q][1]] + TT * TT * lo[perm[q][2]]];
-        const double x = (4.0 * kv[0] - 2.0 * kv[1] - 2.0 * kv[2] - 2.0 * kv[3] + kv[4] + kv[5]) /
-                         (eijk - esh[0][la] - esh[1][lb] - esh[2][lc]);
-        const double kk = p.KU ? p.KU[(long)a + v * b + v * v * c] : kv[0];
-        esum += kk * x;
-        rs[h][0] = x * wsh[0][lb + TT * lc];   // IntX[a,i]
-        rs[h][1] = x * wsh[1][la + TT * lc];   // IntX[b,j]
-        rs[h][2] = x * wsh[2][la + TT * lb];   // IntX[c,k]
-        rs[h][3] = x * wsh[3][lb + TT * lc];   // IntY[a,i]
-        rs[h][4] = x * wsh[4][la + TT * lc];   // IntY[b,j]
-        rs[h][5] = x * wsh[5][la + TT * lb];   // IntY[c,k]
-      }
-    }
-    // fixed-order reductions: a-indexed vectors over (lb, lc), b-indexed over (la, lc),
-    // c-indexed over (la, lb); lanes by xor trees, waves through LDS
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const int role = q % 3;
-      if (role == 0) {
-        double s = rs[0][q] + rs[1][q];
-        s += __shfl_xor(s, 8, 64);
-        s += __shfl_xor(s, 16, 64);
-        s += __shfl_xor(s, 32, 64);
-        if (lb == 0) pab[q][w][la] = s;
-      } else if (role == 1) {
-        double s = rs[0][q] + rs[1][q];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        s += __shfl_xor(s, 4, 64);
-        if (la == 0) pab[q][w][lb] = s;
-      } else {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          double s = rs[h][q];
-          for (int m = 1; m < 64; m <<= 1) s += __shfl_xor(s, m, 64);
-          if (lane == 0) pcc[q][w + 4 * h] = s;
+        for (int q = 0; q < 6; ++q) kv[q] = box[slot[q]][lo[perm[q][0]] + BX1 * lo[perm[q][1]] + BX2 * lo[perm[q][2]]];
+        const KT num = KT(4) * kv[0] - KT(2) * (kv[1] + kv[2] + kv[3]) + kv[4] + kv[5];
+        xv = num / KT(eijk - ea[la] - eb[lb] - ec[lc]);
+        double kk = (double)kv[0];
+        if (p.KU[0]) {
+          const long off = p.boxed ? ((long)tg[0] + nt * ((long)tg[1] + nt * (long)tg[2])) * (TT * TT * TT) + la +
+                                         TT * lb + TT * TT * lc
+                                   : (long)a + v * b + v * v * c;
+          kk = (double)p.KU[0][off] + (double)p.KU[1][off] + (double)p.KU[2][off];
         }
+        esum += kk * (double)xv;
       }
+      xs[la + BX1 * lb + BX2 * lc] = xv;
     }
-    for (int m = 1; m < 64; m <<= 1) esum += __shfl_xor(esum, m, 64);
-    if (lane == 0) ered[w] = esum;
-    __syncthreads();
+    __syncthreads();   // X staged
     double* out = p.part + ((long)tg[0] + (long)nt * (tg[1] + (long)nt * tg[2])) * NPART;
-    if (t < 6 * TT) {
-      const int q = t / TT, l = t % TT;
-      out[t] = (q % 3 == 2) ? pcc[q][l] : ((pab[q][0][l] + pab[q][1][l]) + (pab[q][2][l] + pab[q][3][l]));
+    if (t < 192) {
+      // entry didx of vector dvec: role 0 sums over (b,c), role 1 over (a,c), role 2 over (a,b)
+      const int sl = drole == 0 ? pair(perm[o][1], perm[o][2])
+                                : (drole == 1 ? pair(perm[o][0], perm[o][2]) : pair(perm[o][0], perm[o][1]));
+      const double* ws = wsl[dwy][drole][sl];
+      double sm = 0.0;
+#pragma unroll
+      for (int u2 = 0; u2 < 2; ++u2)
+#pragma unroll
+        for (int u1 = 0; u1 < TT; ++u1) {
+          const int u = 2 * dpart + u2;   // the slower summed index: this part's two planes
+          int xi, wi;
+          if (drole == 0) { xi = didx + BX1 * u1 + BX2 * u; wi = u1 + TT * u; }
+          else if (drole == 1) { xi = u1 + BX1 * didx + BX2 * u; wi = u1 + TT * u; }
+          else { xi = u1 + BX1 * u + BX2 * didx; wi = u1 + TT * u; }
+          sm += (double)xs[xi] * ws[wi];
+        }
+      sm += __shfl_xor(sm, 1, 64);
+      sm += __shfl_xor(sm, 2, 64);
+      if (dpart == 0) out[dvec * TT + didx] = sm;
+    } else if (t == 192 && nb > 0) {
+      out[48] = 0.0;   // the triple's energy is recorded with its first output block
     }
-    if (t == 0) out[48] = (ered[0] + ered[1]) + (ered[2] + ered[3]);
-    __syncthreads();   // slices and partial buffers are reused by the next output block
+    __syncthreads();   // xs is reused by the next output block
+  }
+  // energy of the triple: lanes, then waves, in a fixed order
+  for (int m = 1; m < 64; m <<= 1) esum += __shfl_xor(esum, m, 64);
+  if (lane == 0) ered[w] = esum;
+  __syncthreads();
+  if (t == 0) {
+    const int tg0 = T[0], tg1 = T[1], tg2 = T[2];
+    p.part[((long)tg0 + (long)nt * (tg1 + (long)nt * tg2)) * NPART + 48] = (ered[0] + ered[1]) + (ered[2] + ered[3]);
   }
 }
 
@@ -182,7 +207,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs p) {
 // ordered.  Workgroups g >= 2*ntile: the energy partials of the c-tile slab
 // ct = g - 2*ntile, added to eacc[ct].  Every element has one writer per
 // launch and launches are stream-ordered, so the result is deterministic.
-constexpr int RED_T = 256;
+constexpr int RED_T = 1024;   // 128 strands per vector element: enough loads in flight for the nt^3 records
 __global__ __launch_bounds__(RED_T) void triples_reduce_kernel(int nv, int ntile, int i, int j, int k, double w,
                                                                const double* __restrict__ part, double* IntX,
                                                                double* IntY, double* eacc) {
@@ -191,7 +216,7 @@ __global__ __launch_bounds__(RED_T) void triples_reduce_kernel(int nv, int ntile
   const long nt2 = (long)ntile * ntile;
   if (g < 2 * ntile) {
     const int type = g / ntile, tx = g % ntile;
-    const int l = t & (TT - 1), s8 = t / TT;      // 32 strands per vector element
+    const int l = t & (TT - 1), s8 = t / TT;      // RED_T / TT strands per vector element
     constexpr int NS = RED_T / TT;
     for (int role = 0; role < 3; ++role) {
       const int q = 3 * type + role;
@@ -248,10 +273,11 @@ struct TripleInts {
 
 // K[a,b,c] for triplet (i,j,k) from amplitudes T (v,v,o,o): three segmented
 // GEMM launches, one per output orientation (cc_triples.jl:83-111 lists the
-// twelve products).  Term numbers below follow the order of the particle terms
-// (1-6) and hole terms (7-12) of that loop.
+// twelve products), each writing its own part K[0], K[1], K[2] (K is their sum;
+// no launch reads C, so no epilogue waits on memory).  Term numbers below follow
+// the order of the particle terms (1-6) and hole terms (7-12) of that loop.
 void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const double* T, const TripleInts& I,
-             double* K) {
+             double* const K[3]) {
   const long vv = (long)v * v, vvv = vv * v, vo = (long)v * o, vvo = vv * o;
   auto Tp = [&](int p, int q) { return T + vv * (p + (long)o * q); };          // T[:,:,p,q]
   auto Wm = [&](int m) { return I.W + vvv * m; };                               // W[:,:,:,m]
@@ -268,7 +294,7 @@ void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const do
     seg(op.seg[1], Tp(i, k), v, Wtm(j), vv, v);       // 3: T^{ik}[a,d] W_j[c,b,d]
     seg(op.seg[2], Hn(k, i), v, T + vv * j, vvo, o);  // 11: -H[l,a,k,i] T[b,c,j,l]
     seg(op.seg[3], Hn(j, i), v, T + vvo * k, vv, o);  // 12: -H[l,a,j,i] T[b,c,l,k]
-    op.C = K; op.c_ms0 = 1; op.c_ns0 = v;
+    op.C = K[0]; op.c_ms0 = 1; op.c_ns0 = v;
     op.alpha = 1.0; op.beta = 0.0;
     g.run_prec(op, pr);
   }
@@ -280,8 +306,8 @@ void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const do
     seg(op.seg[1], Wtm(i), vv, Tp(j, k), v, v);       // 5: W_i[c,a,d] T^{jk}[b,d]
     seg(op.seg[2], T + vv * i, vvo, Hn(k, j), v, o);  // 9: T[a,c,i,l] (-H[l,b,k,j])
     seg(op.seg[3], T + vvo * k, vv, Hn(i, j), v, o);  // 10: T[a,c,l,k] (-H[l,b,i,j])
-    op.C = K; op.c_ms0 = 1; op.c_md0 = v; op.c_ms1 = vv; op.c_ns0 = v;
-    op.alpha = 1.0; op.beta = 1.0;
+    op.C = K[1]; op.c_ms0 = 1; op.c_md0 = v; op.c_ms1 = vv; op.c_ns0 = v;
+    op.alpha = 1.0; op.beta = 0.0;
     g.run_prec(op, pr);
   }
   // ---- class c: m = (a,b) contiguous, n = c
@@ -292,17 +318,100 @@ void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const do
     seg(op.seg[1], Wtm(i), vv, Tp(k, j), v, v);       // 6: W_i[b,a,d] T^{kj}[c,d]
     seg(op.seg[2], T + vv * i, vvo, Hn(j, k), v, o);  // 7: T[a,b,i,l] (-H[l,c,j,k])
     seg(op.seg[3], T + vvo * j, vv, Hn(i, k), v, o);  // 8: T[a,b,l,j] (-H[l,c,i,k])
-    op.C = K; op.c_ms0 = 1; op.c_ns0 = vv;
-    op.alpha = 1.0; op.beta = 1.0;
+    op.C = K[2]; op.c_ms0 = 1; op.c_ns0 = vv;
+    op.alpha = 1.0; op.beta = 0.0;
     g.run_prec(op, pr);
   }
+}
+
+// BF16 hi/lo planes of the (T) operands, packed once per evaluation so that each
+// triplet's three products read them directly (no per-triplet packing).  Every
+// family is a stack of blocks with K-contiguous rows:
+//   W [m][(x,y)][d] = W[x,y,d,m]     Wt[m][(x,y)][d] = W[y,x,d,m]   (K = v)
+//   T3[x][(p,q)][l] = T[p,q,x,l]     T4[x][(p,q)][l] = T[p,q,l,x]   (K = o)
+//   Ts[(i,j)][a][d] = T[a,d,i,j]                                     (K = v)
+//   Hn[(m,n)][c][l] = -H[l,c,m,n]                                    (K = o)
+struct Planes {
+  DevBuf* buf = nullptr;     // storage owned by the context (grown on demand)
+  uint16_t* hi = nullptr;
+  uint16_t* lo = nullptr;
+  long ld = 0;
+  // rows of src as in GemmEngine::pack_rows_bf16x3; 256 readable rows are kept after the last
+  void make(GemmEngine& g, const double* src, long rows, int d0, long s0, long s1, int K, long sk) {
+    ld = ((K + 31) / 32) * 32;
+    const long rows_alloc = ((rows + 63) / 64) * 64 + 256;
+    buf->ensure(sizeof(uint16_t) * 2 * rows_alloc * ld);
+    hi = static_cast<uint16_t*>(buf->p);
+    lo = hi + rows_alloc * ld;
+    const long packed = ((rows + 63) / 64) * 64;
+    ECG_HIP(hipMemsetAsync(hi + packed * ld, 0, sizeof(uint16_t) * (rows_alloc - packed) * ld, g.stream));
+    ECG_HIP(hipMemsetAsync(lo + packed * ld, 0, sizeof(uint16_t) * (rows_alloc - packed) * ld, g.stream));
+    g.pack_rows_bf16x3(src, rows, d0, s0, s1, K, sk, (int)ld, hi, lo);
+  }
+};
+struct PackedInts { Planes W, Wt, Hn; };
+struct PackedAmps { Planes T3, T4, Ts; };
+
+void pack_ints(GemmEngine& g, int v, int o, const TripleInts& I, PackedInts& P) {
+  const long vv = (long)v * v, vvv = vv * v;
+  P.W.make(g, I.W, (long)o * vv, (int)vv, 1, vvv, v, vv);
+  P.Wt.make(g, I.Wt, (long)o * vv, (int)vv, 1, vvv, v, vv);
+  P.Hn.make(g, I.Hn, (long)o * o * v, v, 1, (long)v * o, o, v);
+}
+
+void pack_amps(GemmEngine& g, int v, int o, const double* T, PackedAmps& P) {
+  const long vv = (long)v * v;
+  P.T3.make(g, T, (long)o * vv, 1 << 30, 1, 0, o, vv * o);
+  P.T4.make(g, T, (long)o * vv, (int)vv, 1, vv * o, o, vv);
+  P.Ts.make(g, T, (long)o * o * v, v, 1, vv, v, v);
+}
+
+// K[a,b,c] (FP32) for triplet (i,j,k) from packed planes: the same twelve products
+// as build_K, each class with the pair of K indices carried by the integral (or
+// amplitude pair) operand as M (v^2 rows) and the single index as N (v), so
+// all three launches have the shape v^2 x v x (2v + 2o).  K is written in 8x8x8
+// boxes (the combine's unit), one contiguous 512-element run per box.
+void build_K_packed(GemmEngine& g, int v, int o, int i, int j, int k, const PackedAmps& T, const PackedInts& I,
+                    float* const K[3]) {
+  const long vv = (long)v * v;
+  auto rowsW = [&](const Planes& P, int m) { return m * vv * P.ld; };           // block m of a v^2-row family
+  auto rowsS = [&](const Planes& P, int p, int q) { return (p + (long)o * q) * v * P.ld; };   // v-row block
+  auto seg = [&](PackedSeg& s, const Planes& A, long aoff, const Planes& B, long boff) {
+    s.Ah = A.hi + aoff; s.Al = A.lo + aoff; s.lda = A.ld;
+    s.Bh = B.hi + boff; s.Bl = B.lo + boff; s.ldb = B.ld;
+    s.K = (int)A.ld;
+  };
+  PackedOp op;
+  op.M = (int)vv; op.N = v; op.nseg = 4; op.Cf = K[0];
+  op.algorithmic_k = 2LL * v + 2LL * o;
+  // class a: m = (b,c), n = a; K[a,b,c] at a + v*(b + v c)
+  seg(op.seg[0], I.W, rowsW(I.W, k), T.Ts, rowsS(T.Ts, i, j));      // 1: T^{ij}[a,d] W_k[b,c,d]
+  seg(op.seg[1], I.Wt, rowsW(I.Wt, j), T.Ts, rowsS(T.Ts, i, k));    // 3: T^{ik}[a,d] W_j[c,b,d]
+  seg(op.seg[2], T.T3, rowsW(T.T3, j), I.Hn, rowsS(I.Hn, k, i));    // 11: -H[l,a,k,i] T[b,c,j,l]
+  seg(op.seg[3], T.T4, rowsW(T.T4, k), I.Hn, rowsS(I.Hn, j, i));    // 12: -H[l,a,j,i] T[b,c,l,k]
+  op.box_class = 1; op.box_v = v; op.beta = 0.0;
+  g.run_bf16x3_packed(op);
+  // class b: m = (a,c), n = b; K at a + v^2 c + v b
+  seg(op.seg[0], I.W, rowsW(I.W, k), T.Ts, rowsS(T.Ts, j, i));      // 2: W_k[a,c,d] T^{ji}[b,d]
+  seg(op.seg[1], I.Wt, rowsW(I.Wt, i), T.Ts, rowsS(T.Ts, j, k));    // 5: W_i[c,a,d] T^{jk}[b,d]
+  seg(op.seg[2], T.T3, rowsW(T.T3, i), I.Hn, rowsS(I.Hn, k, j));    // 9: T[a,c,i,l] (-H[l,b,k,j])
+  seg(op.seg[3], T.T4, rowsW(T.T4, k), I.Hn, rowsS(I.Hn, i, j));    // 10: T[a,c,l,k] (-H[l,b,i,j])
+  op.box_class = 2; op.Cf = K[1];
+  g.run_bf16x3_packed(op);
+  // class c: m = (a,b), n = c; K at (a + v b) + v^2 c
+  seg(op.seg[0], I.W, rowsW(I.W, j), T.Ts, rowsS(T.Ts, k, i));      // 4: W_j[a,b,d] T^{ki}[c,d]
+  seg(op.seg[1], I.Wt, rowsW(I.Wt, i), T.Ts, rowsS(T.Ts, k, j));    // 6: W_i[b,a,d] T^{kj}[c,d]
+  seg(op.seg[2], T.T3, rowsW(T.T3, i), I.Hn, rowsS(I.Hn, j, k));    // 7: T[a,b,i,l] (-H[l,c,j,k])
+  seg(op.seg[3], T.T4, rowsW(T.T4, j), I.Hn, rowsS(I.Hn, i, k));    // 8: T[a,b,l,j] (-H[l,c,i,k])
+  op.box_class = 3; op.Cf = K[2];
+  g.run_bf16x3_packed(op);
 }
 
 // Transposed and negated-transposed integral copies for build_K.
 void make_triple_ints(TensorOps& ops, int v, int o, const double* W, const double* H, DevBuf& Wt, DevBuf& Hn) {
   const int p_yx[4] = {1, 0, 2, 3};
-  Wt.alloc(sizeof(double) * (long)v * v * v * o);
-  Hn.alloc(sizeof(double) * (long)v * o * o * o);
+  Wt.ensure(sizeof(double) * (long)v * v * v * o);
+  Hn.ensure(sizeof(double) * (long)v * o * o * o);
   Tensor tw = T4d(const_cast<double*>(W), v, v, v, o), twt = T4d(Wt.d(), v, v, v, o);
   ops.permute(tw, p_yx, twt, 1.0, 0.0);
   Tensor th = T4d(const_cast<double*>(H), o, v, o, o), thn = T4d(Hn.d(), v, o, o, o);
@@ -320,11 +429,19 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   TensorOps& ops = c.ops;
   ops.gemm.stream = s;
   const Blocks& Rb = c.hermitian ? c.L : c.R;
-  DevBuf dT1(sizeof(double) * n1), dT2(sizeof(double) * n2);
-  ECG_HIP(hipMemcpyAsync(dT1.p, T1h, dT1.bytes, hipMemcpyHostToDevice, s));
-  ECG_HIP(hipMemcpyAsync(dT2.p, T2h, dT2.bytes, hipMemcpyHostToDevice, s));
+  // the large buffers live in the context between calls (hipMalloc of tens of GB
+  // costs far more than a triplet); every size below is explicit
+  DevBuf* W = c.tws;
+  DevBuf dT1(sizeof(double) * n1);
+  DevBuf& dT2 = W[0];
+  dT2.ensure(sizeof(double) * n2);
+  ECG_HIP(hipMemcpyAsync(dT1.p, T1h, sizeof(double) * n1, hipMemcpyHostToDevice, s));
+  ECG_HIP(hipMemcpyAsync(dT2.p, T2h, sizeof(double) * n2, hipMemcpyHostToDevice, s));
   // undressed integrals: vvvo[a,b,c,k] = (ac|bk),  ovoo[l,c,j,k] = (lj|ck)
-  DevBuf vvvo(sizeof(double) * vvv * o), ovoo(sizeof(double) * (long)o * v * o * o);
+  DevBuf& vvvo = W[1];
+  DevBuf& ovoo = W[2];
+  vvvo.ensure(sizeof(double) * vvv * o);
+  ovoo.ensure(sizeof(double) * (long)o * v * o * o);
   {
     Tensor tw = T4d(vvvo.d(), v, v, v, o), th = T4d(ovoo.d(), o, v, o, o);
     ops.arena.reset();
@@ -333,27 +450,34 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
     ops.contract_p(Prec::F64, 1.0, T3d(c.L.oo.d(), o, o, c.nq), "ljQ", T3d(Rb.vo.d(), v, o, c.nq), "ckQ", 0.0, th,
                    "lcjk");
   }
-  DevBuf vvvoT, ovooN;
+  DevBuf& vvvoT = W[3];
+  DevBuf& ovooN = W[4];
   make_triple_ints(ops, v, o, vvvo.d(), ovoo.d(), vvvoT, ovooN);
   const TripleInts IT{vvvo.d(), vvvoT.d(), ovooN.d()};
   // Lambda: covariant U2 = (2U + U[b,a,i,j])/3 (cctools.jl:935-938) and the
   // integrals <ck|ab> as [a,b,c,k] = (ca|kb), <jk|lc> as [l,c,j,k] = (jl|kc)
-  DevBuf dU1, dU2, vvvoL, ovooL, vvvoLT, ovooLN;
+  DevBuf dU1;
+  DevBuf& dU2 = W[5];
+  DevBuf& vvvoL = W[6];
+  DevBuf& ovooL = W[7];
+  DevBuf& vvvoLT = W[8];
+  DevBuf& ovooLN = W[9];
   TripleInts IL = IT;
   if (lambda) {
     dU1.alloc(sizeof(double) * n1);
-    dU2.alloc(sizeof(double) * n2);
-    ECG_HIP(hipMemcpyAsync(dU1.p, U1h, dU1.bytes, hipMemcpyHostToDevice, s));
-    DevBuf raw(sizeof(double) * n2);
-    ECG_HIP(hipMemcpyAsync(raw.p, U2h, raw.bytes, hipMemcpyHostToDevice, s));
+    dU2.ensure(sizeof(double) * n2);
+    ECG_HIP(hipMemcpyAsync(dU1.p, U1h, sizeof(double) * n1, hipMemcpyHostToDevice, s));
+    DevBuf& raw = W[10];
+    raw.ensure(sizeof(double) * n2);
+    ECG_HIP(hipMemcpyAsync(raw.p, U2h, sizeof(double) * n2, hipMemcpyHostToDevice, s));
     Tensor tr = T4d(raw.d(), v, v, o, o), tu = T4d(dU2.d(), v, v, o, o);
     const int p_ba[4] = {1, 0, 2, 3};
     ops.axpby(n2, 2.0 / 3.0, raw.d(), 0.0, dU2.d());
     ops.permute(tr, p_ba, tu, 1.0 / 3.0, 1.0);
     if (!c.hermitian) {
       // (ca|kb) = (ac|bk) and (jl|kc) = (lj|ck) hold only for Hermitian real integrals
-      vvvoL.alloc(vvvo.bytes);
-      ovooL.alloc(ovoo.bytes);
+      vvvoL.ensure(sizeof(double) * vvv * o);
+      ovooL.ensure(sizeof(double) * (long)o * v * o * o);
       Tensor tw = T4d(vvvoL.d(), v, v, v, o), th = T4d(ovooL.d(), o, v, o, o);
       ops.arena.reset();
       ops.contract_p(Prec::F64, 1.0, T3d(c.L.vv.d(), v, v, c.nq), "caQ", T3d(Rb.ov.d(), o, v, c.nq), "kbQ", 0.0, tw,
@@ -364,10 +488,36 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
       IL = TripleInts{vvvoL.d(), vvvoLT.d(), ovooLN.d()};
     }
   }
+  // BF16 hi/lo products run on operands packed once for all triplets, with K in FP32
+  const bool packed = (pr == Prec::BF16X3);
+  PackedInts PI, PIL;
+  PackedAmps PT, PU;
+  PI.W.buf = &W[11]; PI.Wt.buf = &W[12]; PI.Hn.buf = &W[13];
+  PT.T3.buf = &W[14]; PT.T4.buf = &W[15]; PT.Ts.buf = &W[16];
+  PU.T3.buf = &W[17]; PU.T4.buf = &W[18]; PU.Ts.buf = &W[19];
+  PIL.W.buf = &W[20]; PIL.Wt.buf = &W[21]; PIL.Hn.buf = &W[22];
+  if (packed) {
+    pack_ints(ops.gemm, v, o, IT, PI);
+    pack_amps(ops.gemm, v, o, dT2.d(), PT);
+    if (lambda) {
+      pack_amps(ops.gemm, v, o, dU2.d(), PU);
+      if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, PIL);
+    }
+  }
+  const PackedInts& PIU = (lambda && !c.hermitian) ? PIL : PI;
   const int ntile = (v + TT - 1) / TT;
   const long nblocks = (long)ntile * ntile * ntile;
-  DevBuf K(sizeof(double) * vvv), KU, part(sizeof(double) * nblocks * NPART);
-  if (lambda) KU.alloc(sizeof(double) * vvv);
+  // the packed path stores K in whole 8x8x8 boxes (ntile^3 of them, padding zeroed below)
+  const size_t kbytes = packed ? sizeof(float) * nblocks * TT * TT * TT : sizeof(double) * vvv;
+  DevBuf& K = W[23];
+  DevBuf& KU = W[24];
+  DevBuf part(sizeof(double) * nblocks * NPART);
+  K.ensure(3 * kbytes);
+  if (lambda) KU.ensure(3 * kbytes);
+  if (packed) {
+    ECG_HIP(hipMemsetAsync(K.p, 0, 3 * kbytes, s));
+    if (lambda) ECG_HIP(hipMemsetAsync(KU.p, 0, 3 * kbytes, s));
+  }
   DevBuf IntX(sizeof(double) * n1), IntY(sizeof(double) * n1), eacc(sizeof(double) * ntile);
   ops.fill(n1, 0.0, IntX.d());
   ops.fill(n1, 0.0, IntY.d());
@@ -380,14 +530,62 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   const unsigned ntri = (unsigned)(trih.size() / 3);
   DevBuf tri(sizeof(int) * trih.size());
   ECG_HIP(hipMemcpyAsync(tri.p, trih.data(), tri.bytes, hipMemcpyHostToDevice, s));
-  CombArgs ca{};
+  // K (and KU) as three parts, one per product class
+  char* kb = static_cast<char*>(K.p);
+  char* kub = lambda ? static_cast<char*>(KU.p) : nullptr;
+  double* const Kd[3] = {reinterpret_cast<double*>(kb), reinterpret_cast<double*>(kb + kbytes),
+                         reinterpret_cast<double*>(kb + 2 * kbytes)};
+  float* const Kf[3] = {reinterpret_cast<float*>(kb), reinterpret_cast<float*>(kb + kbytes),
+                        reinterpret_cast<float*>(kb + 2 * kbytes)};
+  double* const KUd[3] = {reinterpret_cast<double*>(kub), reinterpret_cast<double*>(kub + kbytes),
+                          reinterpret_cast<double*>(kub + 2 * kbytes)};
+  float* const KUf[3] = {reinterpret_cast<float*>(kub), reinterpret_cast<float*>(kub + kbytes),
+                         reinterpret_cast<float*>(kub + 2 * kbytes)};
+  CombArgs<double> ca{};
+  {
+    // distinct output blocks per tie pattern, from a representative triple of each
+    const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}, {2, 0, 1}, {1, 2, 0}};
+    const int rep[4][3] = {{0, 1, 2}, {0, 0, 1}, {0, 1, 1}, {0, 0, 0}};
+    for (int pt = 0; pt < 4; ++pt) {
+      const int* T = rep[pt];
+      int nb = 0;
+      for (int o2 = 0; o2 < 6; ++o2) {
+        const int tg[3] = {T[perm[o2][0]], T[perm[o2][1]], T[perm[o2][2]]};
+        bool dup = false;
+        for (int o3 = 0; o3 < o2; ++o3)
+          dup = dup || (T[perm[o3][0]] == tg[0] && T[perm[o3][1]] == tg[1] && T[perm[o3][2]] == tg[2]);
+        if (dup) continue;
+        ca.blk[pt][nb][0] = (signed char)o2;
+        for (int q = 0; q < 6; ++q) {
+          const int w0 = tg[perm[q][0]], w1 = tg[perm[q][1]], w2 = tg[perm[q][2]];
+          int sl = 0;
+          for (int s5 = 5; s5 >= 0; --s5)
+            if (T[perm[s5][0]] == w0 && T[perm[s5][1]] == w1 && T[perm[s5][2]] == w2) sl = s5;
+          ca.blk[pt][nb][1 + q] = (signed char)sl;
+        }
+        ++nb;
+      }
+      ca.nblk[pt] = nb;
+    }
+  }
   ca.nv = v; ca.no = o; ca.ntile = ntile;
   ca.tri = static_cast<const int*>(tri.p);
-  ca.K = K.d(); ca.KU = lambda ? KU.d() : nullptr;
   ca.eo = c.eps.d(); ca.ev = c.eps.d() + o;
   ca.W = c.vvoo.d();
   ca.Y = lambda ? dU2.d() : dT2.d();
   ca.part = part.d();
+  CombArgs<float> cf{};
+  cf.nv = v; cf.no = o; cf.ntile = ntile; cf.tri = ca.tri;
+  cf.eo = ca.eo; cf.ev = ca.ev; cf.W = ca.W; cf.Y = ca.Y; cf.part = ca.part;
+  ca.boxed = 0;
+  cf.boxed = 1;
+  std::copy(&ca.nblk[0], &ca.nblk[0] + 4, &cf.nblk[0]);
+  std::copy(&ca.blk[0][0][0], &ca.blk[0][0][0] + 4 * 6 * 7, &cf.blk[0][0][0]);
+  for (int q = 0; q < 3; ++q) {
+    ca.K[q] = Kd[q]; ca.KU[q] = lambda ? KUd[q] : nullptr;
+    cf.K[q] = Kf[q]; cf.KU[q] = lambda ? KUf[q] : nullptr;
+  }
+  const unsigned cgrid = ntri;   // one workgroup per sorted tile triple
   // loop order of cc_triples.jl:61-71
   long tidx = -1;
   const long last = count < 0 ? (1L << 62) : first + count;
@@ -398,10 +596,17 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
         ++tidx;
         if (tidx < first || tidx >= last) continue;
         double w = (i < j && j < k) ? 2.0 : 1.0;
-        build_K(ops.gemm, pr, v, o, i, j, k, dT2.d(), IT, K.d());
-        if (lambda) build_K(ops.gemm, pr, v, o, i, j, k, dU2.d(), IL, KU.d());
-        ca.i = i; ca.j = j; ca.k = k;
-        hipLaunchKernelGGL(triples_combine_kernel, dim3(ntri), dim3(256), 0, s, ca);
+        if (packed) {
+          build_K_packed(ops.gemm, v, o, i, j, k, PT, PI, Kf);
+          if (lambda) build_K_packed(ops.gemm, v, o, i, j, k, PU, PIU, KUf);
+          cf.i = i; cf.j = j; cf.k = k;
+          hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid), dim3(256), 0, s, cf);
+        } else {
+          build_K(ops.gemm, pr, v, o, i, j, k, dT2.d(), IT, Kd);
+          if (lambda) build_K(ops.gemm, pr, v, o, i, j, k, dU2.d(), IL, KUd);
+          ca.i = i; ca.j = j; ca.k = k;
+          hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid), dim3(256), 0, s, ca);
+        }
         hipLaunchKernelGGL(triples_reduce_kernel, dim3(3 * ntile), dim3(RED_T), 0, s, v, ntile, i, j, k, w,
                            part.d(), IntX.d(), IntY.d(), eacc.d());
         ECG_HIP(hipGetLastError());

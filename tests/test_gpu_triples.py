@@ -14,6 +14,7 @@ from gpu_helpers import requires_gpu
 pytestmark = [pytest.mark.gpu, requires_gpu]
 
 import elemco_amd as E  # noqa: E402
+from elemco_amd.synthetic import conditioned_scale  # noqa: E402
 from oracle import ccsd as O  # noqa: E402
 from oracle import triples as OT  # noqa: E402
 from oracle.synthetic import make_df_system  # noqa: E402
@@ -58,3 +59,36 @@ def test_h2o_ccsd_t_reference_energy():
     eo = O.ccdriver(O.ECInfo.from_fcidump(read_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"))), "ccsd(t)")
     assert abs(e["(T)"] - eo["(T)"]) < 1e-9
     assert abs(e["[T]"] - eo["[T]"]) < 1e-9
+
+
+@pytest.mark.parametrize("cfg", [(5, 19, 72, 0), (4, 130, 100, 6), (6, 41, 90, 8)])
+@pytest.mark.parametrize("prec", [E.ECG_FP32, E.ECG_BF16X3])
+def test_mixed_triples_against_oracle(cfg, prec):
+    """(T) and Lambda-(T) with FP32 or BF16 hi/lo products (FP64 combine and sums).
+
+    Per product term the FP32 path is accurate to ~2^-24 and the BF16 hi/lo path
+    to ~2^-16 relative (the dropped lo*lo term); E[T] is quadratic in K, so the
+    bars are 1e-6 (FP32) and 4e-5 (BF16 hi/lo) relative to the FP64 result, which
+    itself matches the oracle to 1e-10.  The sizes cover ragged tiles of both
+    operand planes (v = 19, 41, 130; several output tiles in n at v = 130).
+    """
+    s = make_df_system(*cfg, scale=0.7 * conditioned_scale(*cfg[:3]))
+    ECo = O.ECInfo.from_df(s)
+    O.calc_fock_matrix(ECo)
+    O.calc_MP2(ECo)
+    T1, T2 = ECo.files["T_vo"], ECo.files["T_vvoo"]
+    et_o, etb_o = OT.calc_pertT_closed_shell(ECo, T1, T2)
+    EC = E.ECInfo.synthetic(*cfg, scale=0.7 * conditioned_scale(*cfg[:3]))
+    et64, etb64 = EC.ctx.pert_triples(T1, T2)
+    assert abs(et64 - et_o) < 1e-10 * max(1.0, abs(et_o))
+    tol = 1e-6 if prec == E.ECG_FP32 else 4e-5
+    et, etb = EC.ctx.pert_triples(T1, T2, precision=prec)
+    assert abs(et - et64) < tol * abs(et64) and abs(etb - etb64) < tol * abs(etb64), (et, et64, etb, etb64)
+    assert et != et64   # the low-precision products were used
+    rng = np.random.default_rng(cfg[3])
+    U1 = 0.5 * T1 + 0.01 * rng.standard_normal(T1.shape)
+    U2 = 0.5 * T2 + 0.01 * rng.standard_normal(T2.shape)
+    U2 = 0.5 * (U2 + U2.transpose(1, 0, 3, 2))
+    lt64, ltb64 = EC.ctx.pert_triples(T1, T2, U1, U2)
+    lt, ltb = EC.ctx.pert_triples(T1, T2, U1, U2, precision=prec)
+    assert abs(lt - lt64) < tol * abs(lt64) and abs(ltb - ltb64) < tol * abs(ltb64), (lt, lt64, ltb, ltb64)
