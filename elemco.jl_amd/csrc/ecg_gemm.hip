@@ -202,34 +202,23 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
   // epilogue
   double* C = p.C + (long)z * p.c_z;  // z-slab for split-K workspaces, batch for direct output
   if (!p.cplain) C = p.C + (long)bz * p.c_z;
-  // per column group: all 16 offsets, then all loads of C, then all stores (a load
-  // issued after a store to C could alias it, which would serialise every element)
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int n = bn + wn * 64 + j * 16 + li;
-    const bool nok = n < p.N;
-    const long cn = !nok ? 0 : (p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1));
-    long off[16];
-    bool ok[16];
+    int n = bn + wn * 64 + j * 16 + li;
+    if (n >= p.N) continue;
+    long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = bm + wm * 64 + i * 16 + Mfma<T>::row(lk, r);
-        const int e = 4 * i + r;
-        ok[e] = nok && m < p.M;
-        off[e] = ok[e] ? (p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1)) + cn : 0;
+        int m = bm + wm * 64 + i * 16 + Mfma<T>::row(lk, r);
+        if (m >= p.M) continue;
+        long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
+        double v = p.alpha * (double)acc[i][j][r];
+        if (p.beta != 0.0) v += p.beta * C[cm + cn];
+        C[cm + cn] = v;
       }
-    double cv[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) cv[e] = (p.beta != 0.0 && ok[e]) ? C[off[e]] : 0.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int e = 4 * i + r;
-        if (ok[e]) C[off[e]] = p.alpha * (double)acc[i][j][r] + (p.beta != 0.0 ? p.beta * cv[e] : 0.0);
-      }
+    }
   }
 }
 

@@ -121,7 +121,10 @@ __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 
 // TO (transposed output): the MFMAs take B as their first operand, so each
 // accumulator holds D^T -- lanes run along m and the four slots along n -- and
 // the epilogue stores stay contiguous when m is the unit-stride output index.
-template <bool TO>
+// MS (multi-segment): slab k0 selects its segment's planes and row strides;
+// without it all slabs read one pair of planes with the row stride kpad and the
+// row offsets stay loop-invariant.
+template <bool TO, bool MS>
 __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   __shared__ u32x4_t As[2][2][XM][4];   // [buffer][plane][row][16-byte chunk]
   __shared__ u32x4_t Bs[2][2][XN][4];
@@ -143,14 +146,17 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   auto gload = [&](Regs& r, int k0) {
     // segment of this slab (uniform; static indices only, so the arguments stay in SGPRs)
     const u16* ah = p.Ah[0]; const u16* al = p.Al[0]; const u16* bh = p.Bh[0]; const u16* bl = p.Bl[0];
-    long lda = p.lda[0], ldb = p.ldb[0];
+    long lda = p.kpad, ldb = p.kpad;
     int kl = k0;
+    if constexpr (MS) {
+      lda = p.lda[0]; ldb = p.ldb[0];
 #pragma unroll
-    for (int q = 1; q < MAXSEG; ++q)
-      if (q < p.nseg && k0 >= p.kb[q]) {
-        ah = p.Ah[q]; al = p.Al[q]; bh = p.Bh[q]; bl = p.Bl[q];
-        lda = p.lda[q]; ldb = p.ldb[q]; kl = k0 - p.kb[q];
-      }
+      for (int q = 1; q < MAXSEG; ++q)
+        if (q < p.nseg && k0 >= p.kb[q]) {
+          ah = p.Ah[q]; al = p.Al[q]; bh = p.Bh[q]; bl = p.Bl[q];
+          lda = p.lda[q]; ldb = p.ldb[q]; kl = k0 - p.kb[q];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int idx = t + 512 * j;
@@ -369,10 +375,8 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   }
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
-  if (x.cplain || (x.c_ms0 == 1 && x.c_ns0 != 1))
-    hipLaunchKernelGGL(gemm_bf16x3_kernel<true>, dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0, stream, x);
-  else
-    hipLaunchKernelGGL(gemm_bf16x3_kernel<false>, dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0, stream, x);
+  hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0,
+                     stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
   if (ksplit > 1) reduce_slabs(op, ksplit);
@@ -432,9 +436,9 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   hipEvent_t e0 = timing_begin();
   // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
   if (op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1))
-    hipLaunchKernelGGL(gemm_bf16x3_kernel<true>, dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
   else
-    hipLaunchKernelGGL(gemm_bf16x3_kernel<false>, dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, true>), dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
 }
