@@ -232,6 +232,34 @@ def gradient_check(E, o, v, q, seed, meth, ndir=20, step=1e-4, thr=1e-20):
     return out
 
 
+def triples_energy(E, o, v, q, seed, meth):
+    """Full (T) per precision on converged amplitudes of the conditioned system of the given sizes.
+
+    The amplitudes come from a defect-corrected BF16x3 solve (it ends at the FP64
+    fixed point, thr 1e-10); every (T) then runs over all triplets, so the energy
+    differences are the complete error of the low-precision products."""
+    from elemco_amd.synthetic import conditioned_scale
+    sc = conditioned_scale(o, v, q)
+    EC = E.ECInfo.synthetic(o, v, q, seed, scale=sc)
+    out = {"system": f"o={o} v={v} naux={q} seed={seed}, factor width 0.05*{sc:.4f}"}
+    t0 = time.perf_counter()
+    e, T1, T2, it, conv = EC.ctx.cc_solve(meth, E.ECG_BF16X3, E.default_opts(maxit=60))
+    out["cc_solve"] = {"E_corr": e[0], "iterations": it, "converged": conv, "seconds": round(time.perf_counter() - t0, 2)}
+    log(f"  solve: E={e[0]:.12f} it={it} conv={conv}")
+    for name, prec in (("fp64", E.ECG_FP64), ("fp32", E.ECG_FP32), ("bf16x3", E.ECG_BF16X3)):
+        t0 = time.perf_counter()
+        et, etb = EC.ctx.pert_triples(T1, T2, precision=prec)
+        dt = time.perf_counter() - t0
+        out[name] = {"E_T": et, "E_T_bracket": etb, "seconds": round(dt, 2)}
+        log(f"  (T) {name}: {et:.12f} [T] {etb:.12f} {dt:.1f} s")
+    for name in ("fp32", "bf16x3"):
+        out[name]["abs_dE_T_vs_fp64"] = abs(out[name]["E_T"] - out["fp64"]["E_T"])
+        out[name]["abs_dE_T_bracket_vs_fp64"] = abs(out[name]["E_T_bracket"] - out["fp64"]["E_T_bracket"])
+        out[name]["speedup_vs_fp64"] = round(out["fp64"]["seconds"] / out[name]["seconds"], 3)
+    EC.ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,6 +278,8 @@ def main():
     ap.add_argument("--gradcheck", action="store_true",
                     help="only the energy-gradient check against finite differences (one JSON object)")
     ap.add_argument("--directions", type=int, default=20, help="random directions of the gradient check")
+    ap.add_argument("--triples-energy", action="store_true",
+                    help="only full (T) per precision on converged amplitudes of the conditioned system (one JSON object)")
     ap.add_argument("--triples-only", action="store_true",
                     help="only the (T) rates per precision on --triplets sampled triplets (one JSON object)")
     args = ap.parse_args()
@@ -269,6 +299,10 @@ def main():
 
     o, v, q, seed = CONFIGS[args.config]
     meth = E.ECG_DCSD if args.method == "dcsd" else E.ECG_CCSD
+    if args.triples_energy:
+        log(f"full (T) per precision o={o} v={v} naux={q}")
+        print(json.dumps(triples_energy(E, o, v, q, seed, meth)), flush=True)
+        return
     if args.gradcheck:
         log(f"gradient check o={o} v={v} naux={q}")
         print(json.dumps(gradient_check(E, o, v, q, seed, meth, args.directions)), flush=True)
