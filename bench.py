@@ -260,6 +260,48 @@ def triples_energy(E, o, v, q, seed, meth):
     return out
 
 
+def batch_study(E, nsys=64, max_threads=0):
+    """The many-system batch: nsys seeded closed-shell CCSD systems (nocc uniform in 4..16,
+    nvirt = 9 nocc, naux = 3 (nocc + nvirt), seeds 100..), solved one after another and as one
+    ecg_cc_solve_batch call (FP64 eager, FP64 with the residual replayed from captured HIP
+    graphs, FP32 products with FP64 defect correction), thr 1e-10.  The first call of each
+    kind is a warm-up (workspace allocation, graph capture on every context)."""
+    import numpy as np
+    from elemco_amd.flops import ccsd_iteration
+    from elemco_amd.synthetic import batch_sizes
+    sizes = batch_sizes(nsys)
+    ecs = [E.ECInfo.synthetic(*sz[:4], scale=sz[4]) for sz in sizes]
+    ctxs = [ec.ctx for ec in ecs]
+    opts = E.default_opts(maxit=100)
+    E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, opts, max_threads)            # warm-up
+    t0 = time.perf_counter()
+    e64, it64, conv64 = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, opts, max_threads)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.cc_solve(E.ECG_CCSD, E.ECG_FP64, opts)
+    t_seq = time.perf_counter() - t0
+    gopts = E.default_opts(maxit=100, graphs=1)
+    E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, gopts, max_threads)           # captures the graphs
+    t0 = time.perf_counter()
+    eg, itg, convg = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, gopts, max_threads)
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e32, it32, conv32 = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP32, opts, max_threads)
+    t_mixed = time.perf_counter() - t0
+    flops = sum(ccsd_iteration(o, v, q) * int(it) for (o, v, q, _, _), it in zip(sizes, it64))
+    dE = np.abs(e32[:, 0] - e64[:, 0])
+    return {"systems": nsys, "sizes": "nocc uniform in 4..16 (seeds 100..), nvirt = 9 nocc, naux = 3 (nocc + nvirt)",
+            "fp64_batch_s": round(t_batch, 3), "fp64_sequential_s": round(t_seq, 3),
+            "batch_speedup_vs_sequential": round(t_seq / t_batch, 3),
+            "fp64_batch_tflops": round(flops / t_batch / 1e12, 3),
+            "fp64_batch_graphs_s": round(t_graph, 3),
+            "max_abs_dE_graphs_vs_eager": float(np.max(np.abs(eg[:, 0] - e64[:, 0]))),
+            "iterations_total": int(it64.sum()), "all_converged": bool(conv64.all() and conv32.all() and convg.all()),
+            "mixed_fp32_batch_s": round(t_mixed, 3), "max_abs_dE_mixed_vs_fp64": float(dE.max()),
+            "per_system_abs_dE": [float(x) for x in dE]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +320,8 @@ def main():
     ap.add_argument("--gradcheck", action="store_true",
                     help="only the energy-gradient check against finite differences (one JSON object)")
     ap.add_argument("--directions", type=int, default=20, help="random directions of the gradient check")
+    ap.add_argument("--batch", action="store_true",
+                    help="only the 64-system batch: sequential vs batched solves, mixed vs FP64 (one JSON object)")
     ap.add_argument("--triples-energy", action="store_true",
                     help="only full (T) per precision on converged amplitudes of the conditioned system (one JSON object)")
     ap.add_argument("--triples-only", action="store_true",
@@ -299,6 +343,10 @@ def main():
 
     o, v, q, seed = CONFIGS[args.config]
     meth = E.ECG_DCSD if args.method == "dcsd" else E.ECG_CCSD
+    if args.batch:
+        log("64-system batch")
+        print(json.dumps(batch_study(E)), flush=True)
+        return
     if args.triples_energy:
         log(f"full (T) per precision o={o} v={v} naux={q}")
         print(json.dumps(triples_energy(E, o, v, q, seed, meth)), flush=True)

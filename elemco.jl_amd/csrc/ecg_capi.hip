@@ -25,7 +25,7 @@ ecg::SolverOpts to_opts(const ecg_solver_opts* o) {
   ecg::SolverOpts s;
   if (!o) return s;
   s.thr = o->thr; s.conven = o->conven; s.shifts = o->shifts; s.shiftp = o->shiftp;
-  s.resthr = o->resthr; s.maxit = o->maxit; s.maxdiis = o->maxdiis; s.refine_every = o->refine_every;
+  s.resthr = o->resthr; s.maxit = o->maxit; s.maxdiis = o->maxdiis; s.refine_every = o->refine_every; s.graphs = o->graphs;
   return s;
 }
 
@@ -67,7 +67,7 @@ extern "C" {
 void ecg_default_opts(ecg_solver_opts* o) {
   if (!o) return;
   o->thr = 1e-10; o->conven = 0.1; o->shifts = 0.15; o->shiftp = 0.2; o->resthr = 10.0;
-  o->maxit = 50; o->maxdiis = 6; o->refine_every = 0;
+  o->maxit = 50; o->maxdiis = 6; o->refine_every = 0; o->graphs = 0;
 }
 
 int ecg_create(ecg_ctx** out, int nocc, int nvirt, int naux, const double* h, const double* BL, const double* BR,
@@ -202,9 +202,21 @@ int ecg_cc_solve_batch(int nsys, ecg_ctx* const* ctxs, int method, int precision
   for (int k = 0; k < nsys; ++k)
     if (!ctxs[k]) return -1;
   std::vector<int> st(nsys, 0);
+  // longest first (cost of an iteration ~ v^4 (o^2 + naux)), so the large systems do not
+  // start last and leave the device to one stream at the end
+  std::vector<int> order(nsys);
+  std::vector<double> cost(nsys);
+  for (int k = 0; k < nsys; ++k) {
+    const ecg::Context& c = ctxs[k]->c;
+    const double v = c.nv;
+    cost[k] = v * v * v * v * ((double)c.no * c.no + c.nq);
+    order[k] = k;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   std::atomic<int> next{0};
   auto worker = [&]() {
-    for (int k = next.fetch_add(1); k < nsys; k = next.fetch_add(1)) {
+    for (int q = next.fetch_add(1); q < nsys; q = next.fetch_add(1)) {
+      const int k = order[q];
       st[k] = guarded(ctxs[k], [&](ecg::Context& c) {
         check_method(method);
         ecg::SolverOpts o = to_opts(opts);

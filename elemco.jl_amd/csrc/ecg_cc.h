@@ -13,6 +13,7 @@ struct SolverOpts {
   double thr = 1e-10, conven = 0.1, shifts = 0.15, shiftp = 0.2, resthr = 10.0;
   int maxit = 50, maxdiis = 6;
   int refine_every = 0;  // mixed precision: FP64 residual every k-th iteration (0 = off)
+  int graphs = 0;        // replay the residual from a captured HIP graph
 };
 
 struct DiisState {
@@ -95,6 +96,21 @@ struct Context {
   double last_residual_ms = 0.0, last_ladder_ms = 0.0;
   int fp64_iterations = 0;   // FP64 iterations of the last solve (mixed precision)
   DevBuf tws[25];            // (T) work buffers: kept between calls, grown on demand
+  // Residual launch sequences captured as HIP graphs (SolverOpts::graphs), one per
+  // (method, precision, buffers); a graph is recaptured when a scratch buffer it
+  // recorded (GEMM workspaces, contraction arena) has been reallocated since.
+  struct ResidualGraph {
+    int method = 0;
+    Prec pr = Prec::F64;
+    const double *t1 = nullptr, *t2 = nullptr;
+    double *r1 = nullptr, *r2 = nullptr;
+    const void *ws = nullptr, *pk = nullptr, *arena = nullptr;
+    size_t wsb = 0, pkb = 0;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<ResidualGraph> rgraphs;
+  bool capturing = false;    // a residual is being recorded (no event records, no host waits)
+  void residual_graph(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d);
   ~Context();
 };
 

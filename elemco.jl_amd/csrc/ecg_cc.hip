@@ -283,9 +283,9 @@ void Context::dress(const double* T1d) {
 // inside the diagonal block are formed twice and weighted by 1/2.
 void Context::ladder(Prec pr, const double* T2x, double* R2r) {
   const Blocks& Rh = hermitian ? Ld : Rd;
-  ECG_HIP(hipEventRecord(ev[2], stream));
+  if (!capturing) ECG_HIP(hipEventRecord(ev[2], stream));
   ladder_df(pr, Ld.vv.d(), Rh.vv.d(), T2x, R2r);
-  ECG_HIP(hipEventRecord(ev[3], stream));
+  if (!capturing) ECG_HIP(hipEventRecord(ev[3], stream));
 }
 
 // out[a,b,i,j] += sum_Q sum_cd Lf[a,c,Q] Rf[b,d,Q] X[c,d,i,j] for b >= (start of a's block),
@@ -336,7 +336,7 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   const long v = nv, o = no, Q = nq;
   residual_calls++;
   ops.arena.reset();
-  ECG_HIP(hipEventRecord(ev[0], stream));
+  if (!capturing) ECG_HIP(hipEventRecord(ev[0], stream));
   dress(T1d);
   const Blocks& Rh = hermitian ? Ld : Rd;
   const Blocks& Ru = hermitian ? L : R;
@@ -450,14 +450,68 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   const int p_sym[4] = {1, 0, 3, 2};
   ops.axpby(nvvoo(), 1.0, R2r, 1.0, R2d);
   ops.permute(tR2r, p_sym, tR2, 1.0, 1.0);
-  ECG_HIP(hipEventRecord(ev[1], stream));
-  ECG_HIP(hipEventSynchronize(ev[1]));
-  float ms = 0.f;
-  ECG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-  last_residual_ms = ms;
-  ECG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-  last_ladder_ms = ms;
+  if (!capturing) {
+    ECG_HIP(hipEventRecord(ev[1], stream));
+    ECG_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    ECG_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    last_residual_ms = ms;
+    ECG_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    last_ladder_ms = ms;
+  }
   ops.arena.reset();
+}
+
+// The residual from a captured HIP graph: the first call for a (method, precision,
+// buffers) key runs eagerly (sizing every scratch buffer and producing this call's
+// result) and then records the same launch sequence; later calls replay it.  The
+// sequence depends only on the sizes, the key's pointers and the scratch buffers,
+// so a replay recomputes the residual of the current T1/T2.
+void Context::residual_graph(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d) {
+  auto scratch = [&](ResidualGraph& g) {
+    g.ws = ops.gemm.ws.p; g.wsb = ops.gemm.ws.bytes;
+    g.pk = ops.gemm.pk.p; g.pkb = ops.gemm.pk.bytes;
+    g.arena = ops.arena.buf.p;
+  };
+  for (size_t k = 0; k < rgraphs.size(); ++k) {
+    ResidualGraph& g = rgraphs[k];
+    if (g.method != method || g.pr != pr || g.t1 != T1d || g.t2 != T2d || g.r1 != R1d || g.r2 != R2d) continue;
+    ResidualGraph now;
+    scratch(now);
+    if (now.ws == g.ws && now.wsb == g.wsb && now.pk == g.pk && now.pkb == g.pkb && now.arena == g.arena) {
+      residual_calls++;
+      ECG_HIP(hipGraphLaunch(g.exec, stream));
+      return;
+    }
+    (void)hipGraphExecDestroy(g.exec);   // a recorded scratch buffer moved: record again
+    rgraphs.erase(rgraphs.begin() + (long)k);
+    break;
+  }
+  residual(method, pr, T1d, T2d, R1d, R2d);
+  ResidualGraph g;
+  g.method = method; g.pr = pr; g.t1 = T1d; g.t2 = T2d; g.r1 = R1d; g.r2 = R2d;
+  scratch(g);
+  hipGraph_t graph = nullptr;
+  capturing = true;
+  ECG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  try {
+    residual(method, pr, T1d, T2d, R1d, R2d);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    capturing = false;
+    throw;
+  }
+  capturing = false;
+  ECG_HIP(hipStreamEndCapture(stream, &graph));
+  ResidualGraph after;
+  scratch(after);
+  ECG_CHECK(after.ws == g.ws && after.pk == g.pk && after.arena == g.arena,
+            "residual graph: a scratch buffer was reallocated during capture");
+  hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  ECG_HIP(e);
+  rgraphs.push_back(g);
 }
 
 // ---------------------------------------------------------------- energies, norms
@@ -682,6 +736,8 @@ void Context::diis_perform(double* A1, double* A2, const double* R1d, const doub
 
 // ---------------------------------------------------------------- iterations
 Context::~Context() {
+  for (auto& g : rgraphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -710,13 +766,17 @@ void Context::state_init(const double* T1h, const double* T2h, const SolverOpts&
 void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info, bool refresh) {
   const long n1 = nvo();
   iter++;
-  residual(method, pr, T1.d(), T2.d(), R1.d(), R2.d());
+  auto resid = [&](Prec p, double* r1, double* r2) {
+    if (o.graphs && !ops.gemm.timed) residual_graph(method, p, T1.d(), T2.d(), r1, r2);
+    else residual(method, p, T1.d(), T2.d(), r1, r2);
+  };
+  resid(pr, R1.d(), R2.d());
   if (pr != Prec::F64) {
     if (refresh) {
       // delta = R_64 - R_low at this T; continue with the exact residual R_low + delta = R_64
       D1.ensure(sizeof(double) * n1);
       D2.ensure(sizeof(double) * nvvoo());
-      residual(method, Prec::F64, T1.d(), T2.d(), D1.d(), D2.d());
+      resid(Prec::F64, D1.d(), D2.d());
       ops.axpby(n1, -1.0, R1.d(), 1.0, D1.d());
       ops.axpby(nvvoo(), -1.0, R2.d(), 1.0, D2.d());
       have_delta = true;

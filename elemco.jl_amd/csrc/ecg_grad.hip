@@ -244,8 +244,7 @@ void energy_grad(Context& c, int method, const double* T1h, const double* T2h, c
   }
   // undressed Fock (energy's f_ov): f = h + 2 L tr_u - sum_k L[p,k] L[k,q] with undressed factors
   Buf UGoo(ops, o * o * Q), UGov(ops, o * v * Q), UGvo(ops, v * o * Q), UGvv(ops, v * v * Q);
-  Tensor ugoo = V3(UGoo.d(), o, o, Q), ugov = V3(UGov.d(), o, v, Q), ugvo = V3(UGvo.d(), v, o, Q),
-         ugvv = V3(UGvv.d(), v, v, Q);
+  Tensor ugoo = V3(UGoo.d(), o, o, Q), ugov = V3(UGov.d(), o, v, Q);
   Tensor uoo = V3(U.oo.d(), o, o, Q), uov = V3(U.ov.d(), o, v, Q);
   {
     Buf utr(ops, Q), gutr(ops, Q);
@@ -269,7 +268,7 @@ void energy_grad(Context& c, int method, const double* T1h, const double* T2h, c
   ops.axpby(o * v * Q, 1.0, Gov.d(), 1.0, UGov.d());
   // dh = (1 - t) h (1 + t) reversed; d L / d h^ = d L / d f^
   Buf Hoo(ops, o * o), Hov(ops, o * v), Hvo(ops, v * o), Hvv(ops, v * v);
-  Tensor hoo = V2(Hoo.d(), o, o), hov = V2(Hov.d(), o, v), hvv = V2(Hvv.d(), v, v);
+  Tensor hoo = V2(Hoo.d(), o, o), hov = V2(Hov.d(), o, v);
   ops.contract_p(F, 1.0, fvo, "ai", t1, "bi", 1.0, fvv, "ab");
   ops.axpby(v * o, 1.0, Fvo.d(), 1.0, Hvo.d());
   ops.contract_p(F, -1.0, t1, "aj", fvo, "ai", 1.0, hoo, "ji");

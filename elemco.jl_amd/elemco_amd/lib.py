@@ -25,7 +25,7 @@ class ElemcoError(RuntimeError):
 class SolverOpts(ctypes.Structure):
     _fields_ = [("thr", ctypes.c_double), ("conven", ctypes.c_double), ("shifts", ctypes.c_double),
                 ("shiftp", ctypes.c_double), ("resthr", ctypes.c_double), ("maxit", ctypes.c_int),
-                ("maxdiis", ctypes.c_int), ("refine_every", ctypes.c_int)]
+                ("maxdiis", ctypes.c_int), ("refine_every", ctypes.c_int), ("graphs", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

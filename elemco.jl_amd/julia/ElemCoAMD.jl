@@ -44,10 +44,11 @@ mutable struct SolverOpts
   maxit::Cint
   maxdiis::Cint
   refine_every::Cint
+  graphs::Cint
 end
 
 function default_opts(; kw...)
-  o = SolverOpts(0, 0, 0, 0, 0, 0, 0, 0)
+  o = SolverOpts(0, 0, 0, 0, 0, 0, 0, 0, 0)
   ccall((:ecg_default_opts, LIB), Cvoid, (Ref{SolverOpts},), o)
   for (k, v) in kw
     setfield!(o, k, convert(fieldtype(SolverOpts, k), v))

@@ -38,6 +38,8 @@ typedef struct {
   int maxit;         /* maximum iterations (50) */
   int maxdiis;       /* DIIS subspace (6) */
   int refine_every;  /* mixed precision: FP64 residual every k-th iteration (0: never) */
+  int graphs;        /* 1: the residual's launch sequence is captured once in a HIP graph and
+                        replayed every iteration (for launch-bound small systems); 0: eager */
 } ecg_solver_opts;
 
 void ecg_default_opts(ecg_solver_opts* opts);

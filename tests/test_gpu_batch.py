@@ -58,3 +58,15 @@ def test_batch_reports_failing_system():
     _, ecs = _systems()
     with pytest.raises(E.ElemcoError, match="system 0"):
         E.cc_solve_batch([ec.ctx for ec in ecs[:2]], 7, E.ECG_FP64)
+
+
+def test_batch_with_graph_replay_matches_eager():
+    """The batch with every context replaying its captured residual graph (opts.graphs = 1,
+    one capture per host thread's stream) gives the eager batch's energies (1e-12 Eh)."""
+    _, ecs = _systems()
+    ctxs = [ec.ctx for ec in ecs]
+    opts = dict(maxit=100, thr=1e-12)
+    e0, it0, c0 = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, E.default_opts(**opts))
+    e1, it1, c1 = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, E.default_opts(graphs=1, **opts))
+    assert c0.all() and c1.all() and (it0 == it1).all()
+    assert float(np.max(np.abs(e1[:, 0] - e0[:, 0]))) < 1e-12

@@ -16,6 +16,7 @@ from gpu_helpers import requires_gpu
 pytestmark = [pytest.mark.gpu, requires_gpu]
 
 import elemco_amd as E  # noqa: E402
+from elemco_amd.synthetic import conditioned_scale  # noqa: E402
 from oracle import ccsd as O  # noqa: E402
 from oracle.synthetic import make_df_system  # noqa: E402
 
@@ -159,3 +160,25 @@ def test_conditioned_c1_energies_and_triples_golden():
     et, etb = EC.ctx.pert_triples(EC.files["T_vo"], EC.files["T_vvoo"])
     assert abs(et - cond["(T)"]) < 1e-9
     assert abs(etb - cond["[T]"]) < 1e-9
+
+
+@pytest.mark.parametrize("method,prec", [(E.ECG_CCSD, E.ECG_FP64), (E.ECG_DCSD, E.ECG_FP64),
+                                         (E.ECG_CCSD, E.ECG_FP32)])
+def test_graph_replayed_residual_matches_eager(method, prec):
+    """Solves with the residual replayed from captured HIP graphs (opts.graphs = 1) give the
+    eager solve's iterations and energies to rounding: every iteration replays the graph with
+    eager updates, DIIS and host reductions in between, and the mixed solve also replays the
+    FP64 refresh residual into the defect buffers.  Bar: 1e-12 Eh, same iteration count."""
+    cfg = (6, 40, 90, 21)
+    EC = E.ECInfo.synthetic(*cfg, scale=0.7 * conditioned_scale(*cfg[:3]))
+    opts = dict(thr=1e-12, maxit=100)
+    e0, T10, T20, it0, conv0 = EC.ctx.cc_solve(method, prec, E.default_opts(**opts))
+    e1, T11, T21, it1, conv1 = EC.ctx.cc_solve(method, prec, E.default_opts(graphs=1, **opts))
+    e2, _, _, it2, conv2 = EC.ctx.cc_solve(method, prec, E.default_opts(graphs=1, **opts))   # cached graphs
+    assert conv0 and conv1 and conv2 and it0 == it1 == it2
+    assert abs(e1[0] - e0[0]) < 1e-12 and abs(e2[0] - e0[0]) < 1e-12
+    assert float(np.max(np.abs(T21 - T20))) < 1e-10
+    # an eager residual between replays (other work on the same stream) leaves them valid
+    R1, R2 = EC.ctx.cc_residual(T10, T20, method)
+    e3, _, _, it3, _ = EC.ctx.cc_solve(method, prec, E.default_opts(graphs=1, **opts))
+    assert it3 == it0 and abs(e3[0] - e0[0]) < 1e-12
