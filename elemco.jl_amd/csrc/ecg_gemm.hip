@@ -57,7 +57,9 @@ template <> struct Mfma<float> {
 
 // T = double: FP64 MFMA.  T = float: operands rounded to FP32 when staged in
 // LDS, FP32 MFMA accumulation, FP64 epilogue (alpha, beta and C stay FP64).
-template <typename T, bool AK, bool BK>
+// SEG: several K segments (the slab's segment is looked up per slab); without
+// it the single product's base pointers and strides stay in registers.
+template <typename T, bool AK, bool BK, bool SEG>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[2][BKS][LDP];
@@ -117,11 +119,13 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
     const double* B = p.sB[0];
     long a_k = p.sak[0], b_k = p.sbk[0];
     int kl = k0, klen = p.sK[0];
+    if constexpr (SEG) {
 #pragma unroll
-    for (int q = 1; q < MAXSEG; ++q) {
-      if (q < p.nseg && k0 >= p.kp[q]) {
-        A = p.sA[q]; B = p.sB[q]; a_k = p.sak[q]; b_k = p.sbk[q];
-        kl = k0 - p.kp[q]; klen = p.sK[q];
+      for (int q = 1; q < MAXSEG; ++q) {
+        if (q < p.nseg && k0 >= p.kp[q]) {
+          A = p.sA[q]; B = p.sB[q]; a_k = p.sak[q]; b_k = p.sbk[q];
+          kl = k0 - p.kp[q]; klen = p.sK[q];
+        }
       }
     }
     A += (long)bz * p.a_z;
@@ -176,23 +180,35 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(GemmParams p) {
     sstore(0);
   }
   __syncthreads();
+  // operand fragments of k-step kk of the current LDS buffer
+  auto frag = [&](T (&a)[4], T (&b)[4], int kk) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = As[cur][kk + lk][wm * 64 + i * 16 + li];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = Bs[cur][kk + lk][wn * 64 + j * 16 + li];
+  };
   for (int k0 = kbeg; k0 < kend; k0 += BKS) {
     const bool more = k0 + BKS < kend;
+    // the first fragments are read before the prefetch, whose address arithmetic then
+    // covers their LDS latency
+    T a[4], b[4];
+    frag(a, b, 0);
     // unconditional prefetch (the last slab reloads itself): a conditional load would make
     // the compiler wait for it before the MFMAs below instead of before the LDS store
     gload(more ? k0 + BKS : k0);
     asm volatile("" ::: "memory");   // keep the prefetch loads issued ahead of the MFMAs
 #pragma unroll
     for (int kk = 0; kk < BKS; kk += 4) {
-      T a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[cur][kk + lk][wm * 64 + i * 16 + li];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[cur][kk + lk][wn * 64 + j * 16 + li];
+      T an[4], bn[4];
+      if (kk + 4 < BKS) frag(an, bn, kk + 4);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+      if (kk + 4 < BKS) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { a[i] = an[i]; b[i] = bn[i]; }
+      }
     }
     if (more) sstore(cur ^ 1);
     __syncthreads();
@@ -238,6 +254,21 @@ __global__ void splitk_reduce_kernel(const double* __restrict__ W, int ksplit, l
   double v = alpha * s;
   if (beta != 0.0) v += beta * (*c);
   *c = v;
+}
+
+template <typename T>
+void launch_gemm(bool ak, bool bk, bool seg, dim3 grid, hipStream_t stream, const GemmParams& p) {
+  if (seg) {
+    if (!ak && !bk) gemm_kernel<T, false, false, true><<<grid, dim3(256), 0, stream>>>(p);
+    else if (ak && !bk) gemm_kernel<T, true, false, true><<<grid, dim3(256), 0, stream>>>(p);
+    else if (!ak && bk) gemm_kernel<T, false, true, true><<<grid, dim3(256), 0, stream>>>(p);
+    else gemm_kernel<T, true, true, true><<<grid, dim3(256), 0, stream>>>(p);
+  } else {
+    if (!ak && !bk) gemm_kernel<T, false, false, false><<<grid, dim3(256), 0, stream>>>(p);
+    else if (ak && !bk) gemm_kernel<T, true, false, false><<<grid, dim3(256), 0, stream>>>(p);
+    else if (!ak && bk) gemm_kernel<T, false, true, false><<<grid, dim3(256), 0, stream>>>(p);
+    else gemm_kernel<T, true, true, false><<<grid, dim3(256), 0, stream>>>(p);
+  }
 }
 
 void GemmEngine::run(const GemmOp& op) { run_prec(op, prec); }
@@ -321,17 +352,8 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     ECG_CHECK((sg[s].a_k == 1 && op.a_s0 != 1) == ak && (sg[s].b_k == 1 && op.b_s0 != 1) == bk,
               "gemm: K segments differ in operand contiguity");
   hipEvent_t e0 = timing_begin();
-  if (pr == Prec::F64) {
-    if (!ak && !bk) gemm_kernel<double, false, false><<<grid, dim3(256), 0, stream>>>(p);
-    else if (ak && !bk) gemm_kernel<double, true, false><<<grid, dim3(256), 0, stream>>>(p);
-    else if (!ak && bk) gemm_kernel<double, false, true><<<grid, dim3(256), 0, stream>>>(p);
-    else gemm_kernel<double, true, true><<<grid, dim3(256), 0, stream>>>(p);
-  } else {
-    if (!ak && !bk) gemm_kernel<float, false, false><<<grid, dim3(256), 0, stream>>>(p);
-    else if (ak && !bk) gemm_kernel<float, true, false><<<grid, dim3(256), 0, stream>>>(p);
-    else if (!ak && bk) gemm_kernel<float, false, true><<<grid, dim3(256), 0, stream>>>(p);
-    else gemm_kernel<float, true, true><<<grid, dim3(256), 0, stream>>>(p);
-  }
+  if (pr == Prec::F64) launch_gemm<double>(ak, bk, p.nseg > 1, grid, stream, p);
+  else launch_gemm<float>(ak, bk, p.nseg > 1, grid, stream, p);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
   if (ksplit > 1) reduce_slabs(op, ksplit);
