@@ -329,6 +329,7 @@ int ecg_stats(ecg_ctx* ctx, double* stats) {
     stats[4] = (double)c.ops.gemm.flops;
     stats[5] = (double)c.fp64_iterations;
     stats[6] = c.ops.gemm.bytes;
+    stats[7] = (double)c.triples_last_batch;
   });
 }
 

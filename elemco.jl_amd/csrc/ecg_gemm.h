@@ -65,6 +65,7 @@ struct PackedSeg {
   const uint16_t* Ah = nullptr; const uint16_t* Al = nullptr; long lda = 0;
   const uint16_t* Bh = nullptr; const uint16_t* Bl = nullptr; long ldb = 0;
   int K = 0;
+  long a_z = 0, b_z = 0;   // element offsets of A and B (both planes) per batch index
 };
 struct PackedOp {
   int M = 0, N = 0, nseg = 0;
@@ -79,6 +80,10 @@ struct PackedOp {
   // + x1%8 + 8 (x2%8) + 64 (x3%8), nt = ceil(box_v / 8).  With m = y1 + box_v y2 the
   // coordinates are (n, y1, y2) for class 1, (y1, n, y2) for 2 and (y1, y2, n) for 3.
   int box_class = 0, box_v = 0;
+  // batch: products z = 0 .. batch-1 read the segments' planes at + z * a_z / b_z and
+  // write C at + z * c_z (elements)
+  int batch = 1;
+  long c_z = 0;
 };
 
 // Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,

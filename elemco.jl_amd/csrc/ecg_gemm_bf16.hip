@@ -101,6 +101,7 @@ struct X3Params {
   int nseg, kb[MAXSEG + 1];
   const u16* Ah[MAXSEG]; const u16* Al[MAXSEG]; long lda[MAXSEG];
   const u16* Bh[MAXSEG]; const u16* Bl[MAXSEG]; long ldb[MAXSEG];
+  long az[MAXSEG], bz[MAXSEG], cbz;   // per batch index (blockIdx.y) offsets of A, B, C
   double* C; float* Cf; long c_ms0, c_ms1, c_ns0, c_ns1, c_z; int c_md0, c_nd0;
   double alpha, beta;
   int cplain; long ldc_ws;
@@ -135,6 +136,7 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
   const int kz = blockIdx.z;
+  const long zb = blockIdx.y;   // batch index (packed products)
   const int kbeg = kz * p.kchunk;
   const int kend = min(p.kpad, kbeg + p.kchunk);
 
@@ -150,12 +152,15 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
     int kl = k0;
     if constexpr (MS) {
       lda = p.lda[0]; ldb = p.ldb[0];
+      long oa = p.az[0], ob = p.bz[0];
 #pragma unroll
       for (int q = 1; q < MAXSEG; ++q)
         if (q < p.nseg && k0 >= p.kb[q]) {
           ah = p.Ah[q]; al = p.Al[q]; bh = p.Bh[q]; bl = p.Bl[q];
           lda = p.lda[q]; ldb = p.ldb[q]; kl = k0 - p.kb[q];
+          oa = p.az[q]; ob = p.bz[q];
         }
+      ah += zb * oa; al += zb * oa; bh += zb * ob; bl += zb * ob;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -251,7 +256,8 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   // (TO: lane row + 16*(col/4), slot col%4).  Per column group: all 16 offsets, then
   // all loads of C, then all stores (a load issued after a store to C could alias it,
   // which would serialise every element).
-  double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0);
+  double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0) + zb * p.cbz;
+  float* Cf = p.Cf ? p.Cf + zb * p.cbz : nullptr;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     long off[16];
@@ -275,16 +281,16 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
         off[e] = ok[e] ? oe : 0;
       }
     const bool rd = p.beta != 0.0;
-    if (p.Cf) {
+    if (Cf) {
       float cv[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? p.Cf[off[e]] : 0.f;
+      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? Cf[off[e]] : 0.f;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int e = 4 * i + r;
-          if (ok[e]) p.Cf[off[e]] = (float)(p.alpha * (double)acc[i][j][r] + (rd ? p.beta * (double)cv[e] : 0.0));
+          if (ok[e]) Cf[off[e]] = (float)(p.alpha * (double)acc[i][j][r] + (rd ? p.beta * (double)cv[e] : 0.0));
         }
     } else {
       double cv[16];
@@ -412,14 +418,18 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
     x.kb[s] = kpad;
     x.Ah[s] = g.Ah; x.Al[s] = g.Al; x.lda[s] = g.lda;
     x.Bh[s] = g.Bh; x.Bl[s] = g.Bl; x.ldb[s] = g.ldb;
+    x.az[s] = g.a_z; x.bz[s] = g.b_z;
     kpad += g.K;
   }
   for (int s = op.nseg; s < MAXSEG; ++s) {
     x.kb[s] = kpad;
     x.Ah[s] = x.Ah[0]; x.Al[s] = x.Al[0]; x.lda[s] = x.lda[0];
     x.Bh[s] = x.Bh[0]; x.Bl[s] = x.Bl[0]; x.ldb[s] = x.ldb[0];
+    x.az[s] = x.az[0]; x.bz[s] = x.bz[0];
   }
   x.kb[MAXSEG] = kpad;
+  ECG_CHECK(op.batch >= 1 && op.batch <= 65535, "gemm: packed batch out of range");
+  x.cbz = op.c_z;
   x.kpad = kpad; x.kchunk = kpad; x.ksplit = 1;
   x.C = op.C; x.Cf = op.Cf; x.cplain = 0;
   x.c_ms0 = op.c_ms0; x.c_ms1 = op.c_ms1; x.c_md0 = op.c_md0;
@@ -428,17 +438,19 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   x.box_class = op.box_class; x.box_v = op.box_v;
   ECG_CHECK(op.box_class == 0 || ((long)op.box_v * op.box_v == op.M && op.box_v == op.N),
             "gemm: a boxed output needs M = box_v^2 and N = box_v");
-  flops += 2LL * op.M * op.N * op.algorithmic_k;
-  bytes += (op.Cf ? 4.0 : 8.0) * (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N +
-           4.0 * (double)(op.M + op.N) * op.algorithmic_k;
+  flops += 2LL * op.M * op.N * op.algorithmic_k * op.batch;
+  bytes += op.batch * ((op.Cf ? 4.0 : 8.0) * (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N +
+                       4.0 * (double)(op.M + op.N) * op.algorithmic_k);
   const long tiles = (long)x.tiles_m * x.tiles_n;
   ECG_CHECK(tiles < (1L << 31), "gemm: too many tiles");
   hipEvent_t e0 = timing_begin();
   // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
   if (op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1))
-    hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), dim3((unsigned)tiles, (unsigned)op.batch, 1), dim3(512), 0,
+                       stream, x);
   else
-    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, true>), dim3((unsigned)tiles, 1, 1), dim3(512), 0, stream, x);
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, true>), dim3((unsigned)tiles, (unsigned)op.batch, 1), dim3(512), 0,
+                       stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
 }

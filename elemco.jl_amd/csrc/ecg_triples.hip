@@ -43,6 +43,7 @@ struct CombArgs {
   const double* Y;      // T2 (or covariant U2) [a,b,i,j] for IntY
   const int* tri;       // sorted tile triples (A <= B <= C), 3 ints each
   int boxed;            // K (and KU) stored in 8x8x8 boxes (GemmEngine boxed output), else [a,b,c]
+  long kz, pz;          // batch member blockIdx.y: triplet (i + y, j, k), K/KU at + y kz, records at + y pz
   double* part;         // [ntile^3][NPART], record of output block at + nt*(bt + nt*ct)
   // per tie pattern (A==B) + 2 (B==C): the distinct output blocks, each as its
   // permutation index and the box slot of each of the six permutations of K
@@ -80,13 +81,19 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int la = lane & 7, lb = lane >> 3;   // element e = la + 8 lb + 64 lc, lc = w and w + 4
   const int T[3] = {p.tri[3 * blockIdx.x], p.tri[3 * blockIdx.x + 1], p.tri[3 * blockIdx.x + 2]};
+  const long zb = blockIdx.y;
+  const int pi = p.i + (int)zb;
+  const KT* K0 = p.K[0] + zb * p.kz;
+  const KT* K1 = p.K[1] + zb * p.kz;
+  const KT* K2 = p.K[2] + zb * p.kz;
+  double* const part = p.part + zb * p.pz;
   // permutation table: which of the three coordinates sits at K index 1,2,3
   const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}, {2, 0, 1}, {1, 2, 0}};
   // ordered pair (p, q), p != q, of tile positions -> 0..5
   auto pair = [](int a, int b) { return 2 * a + (b > a ? b - 1 : b); };
   // occupied pair of each role: 0 -> (j,k), 1 -> (i,k), 2 -> (i,j)
-  const long ooff[3] = {v * v * (p.j + (long)p.no * p.k), v * v * (p.i + (long)p.no * p.k),
-                        v * v * (p.i + (long)p.no * p.j)};
+  const long ooff[3] = {v * v * (p.j + (long)p.no * p.k), v * v * (pi + (long)p.no * p.k),
+                        v * v * (pi + (long)p.no * p.j)};
   // ---- staging (all global loads of the workgroup)
 #pragma unroll
   for (int s = 0; s < 6; ++s)
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
       else
         off = ok ? (long)x1 + v * x2 + v * v * x3 : 0;
       box[s][e % TT + BX1 * ((e / TT) % TT) + BX2 * (e / (TT * TT))] =
-          ok ? p.K[0][off] + p.K[1][off] + p.K[2][off] : KT(0);
+          ok ? K0[off] + K1[off] + K2[off] : KT(0);
     }
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
     const int x = T[t / TT] * TT + t % TT;
     esl[t / TT][t % TT] = x < v ? p.ev[x < v ? x : 0] : 0.0;
   }
-  const double eijk = p.eo[p.i] + p.eo[p.j] + p.eo[p.k];
+  const double eijk = p.eo[pi] + p.eo[p.j] + p.eo[p.k];
   __syncthreads();
 
   // dot-product assignment of the 192 first threads: entry q = t/4 (vector q/8, index q%8), part t%4
@@ -157,14 +164,14 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
           const long off = p.boxed ? ((long)tg[0] + nt * ((long)tg[1] + nt * (long)tg[2])) * (TT * TT * TT) + la +
                                          TT * lb + TT * TT * lc
                                    : (long)a + v * b + v * v * c;
-          kk = (double)p.KU[0][off] + (double)p.KU[1][off] + (double)p.KU[2][off];
+          kk = (double)p.KU[0][off + zb * p.kz] + (double)p.KU[1][off + zb * p.kz] + (double)p.KU[2][off + zb * p.kz];
         }
         esum += kk * (double)xv;
       }
       xs[la + BX1 * lb + BX2 * lc] = xv;
     }
     __syncthreads();   // X staged
-    double* out = p.part + ((long)tg[0] + (long)nt * (tg[1] + (long)nt * tg[2])) * NPART;
+    double* out = part + ((long)tg[0] + (long)nt * (tg[1] + (long)nt * tg[2])) * NPART;
     if (t < 192) {
       // entry didx of vector dvec: role 0 sums over (b,c), role 1 over (a,c), role 2 over (a,b)
       const int sl = drole == 0 ? pair(perm[o][1], perm[o][2])
@@ -196,66 +203,74 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
   __syncthreads();
   if (t == 0) {
     const int tg0 = T[0], tg1 = T[1], tg2 = T[2];
-    p.part[((long)tg0 + (long)nt * (tg1 + (long)nt * tg2)) * NPART + 48] = (ered[0] + ered[1]) + (ered[2] + ered[3]);
+    part[((long)tg0 + (long)nt * (tg1 + (long)nt * tg2)) * NPART + 48] = (ered[0] + ered[1]) + (ered[2] + ered[3]);
   }
 }
 
-// Reduce the per-block partials of one triplet in a fixed order.
-// Workgroups g < 2*ntile: vector type = g / ntile (0: IntX, 1: IntY) and virtual
-// tile tx = g % ntile; the workgroup adds the i, j and k contributions (roles
-// 0, 1, 2) in turn, so with i = j or j = k the two updates of one element are
-// ordered.  Workgroups g >= 2*ntile: the energy partials of the c-tile slab
-// ct = g - 2*ntile, added to eacc[ct].  Every element has one writer per
-// launch and launches are stream-ordered, so the result is deterministic.
+// Reduce the per-block partials of the triplets (i0 + z, j, k), z < nb, in a fixed
+// order: member by member, and per member as follows.  Workgroups g < 2*ntile:
+// vector type = g / ntile (0: IntX, 1: IntY) and virtual tile tx = g % ntile; the
+// workgroup adds the i, j and k contributions (roles 0, 1, 2) in turn, so with
+// i = j or j = k the two updates of one element are ordered.  Workgroups
+// g >= 2*ntile: the energy partials of the c-tile slab ct = g - 2*ntile, added to
+// eacc[ct].  Every element has one writer per launch and launches are
+// stream-ordered, so the result is deterministic.
 constexpr int RED_T = 1024;   // 128 strands per vector element: enough loads in flight for the nt^3 records
-__global__ __launch_bounds__(RED_T) void triples_reduce_kernel(int nv, int ntile, int i, int j, int k, double w,
-                                                               const double* __restrict__ part, double* IntX,
-                                                               double* IntY, double* eacc) {
+__global__ __launch_bounds__(RED_T) void triples_reduce_kernel(int nv, int ntile, int i0, int nb, int j, int k,
+                                                               const double* __restrict__ part, long pz,
+                                                               double* IntX, double* IntY, double* eacc) {
   __shared__ double red[RED_T];
   const int g = blockIdx.x, t = threadIdx.x;
   const long nt2 = (long)ntile * ntile;
-  if (g < 2 * ntile) {
-    const int type = g / ntile, tx = g % ntile;
-    const int l = t & (TT - 1), s8 = t / TT;      // RED_T / TT strands per vector element
-    constexpr int NS = RED_T / TT;
-    for (int role = 0; role < 3; ++role) {
-      const int q = 3 * type + role;
-      double s = 0.0;
-      for (long u = s8; u < nt2; u += NS) {
-        const int u1 = (int)(u % ntile), u2 = (int)(u / ntile);
-        int at, bt, ct;
-        if (role == 0) { at = tx; bt = u1; ct = u2; }
-        else if (role == 1) { bt = tx; at = u1; ct = u2; }
-        else { ct = tx; at = u1; bt = u2; }
-        const long blk = at + (long)ntile * (bt + (long)ntile * ct);
-        s += part[blk * NPART + q * TT + l];
-      }
-      red[t] = s;
-      __syncthreads();
-      for (int h = NS / 2; h > 0; h >>= 1) {
-        if (s8 < h) red[t] += red[t + h * TT];
+  // the members (i0 + z, j, k) of a batch, in order z = 0, 1, ...
+  for (int z = 0; z < nb; ++z) {
+    const int i = i0 + z;
+    const double w = (i < j && j < k) ? 2.0 : 1.0;
+    const double* pm = part + (long)z * pz;
+    if (g < 2 * ntile) {
+      const int type = g / ntile, tx = g % ntile;
+      const int l = t & (TT - 1), s8 = t / TT;      // RED_T / TT strands per vector element
+      constexpr int NS = RED_T / TT;
+      for (int role = 0; role < 3; ++role) {
+        const int q = 3 * type + role;
+        double s = 0.0;
+        for (long u = s8; u < nt2; u += NS) {
+          const int u1 = (int)(u % ntile), u2 = (int)(u / ntile);
+          int at, bt, ct;
+          if (role == 0) { at = tx; bt = u1; ct = u2; }
+          else if (role == 1) { bt = tx; at = u1; ct = u2; }
+          else { ct = tx; at = u1; bt = u2; }
+          const long blk = at + (long)ntile * (bt + (long)ntile * ct);
+          s += pm[blk * NPART + q * TT + l];
+        }
+        red[t] = s;
+        __syncthreads();
+        for (int h = NS / 2; h > 0; h >>= 1) {
+          if (s8 < h) red[t] += red[t + h * TT];
+          __syncthreads();
+        }
+        const int x = tx * TT + l;
+        if (s8 == 0 && x < nv) {
+          const int occ = role == 0 ? i : (role == 1 ? j : k);
+          double* dst = (type == 0 ? IntX : IntY) + x + (long)nv * occ;
+          *dst += w * red[t];
+        }
         __syncthreads();
       }
-      const int x = tx * TT + l;
-      if (s8 == 0 && x < nv) {
-        const int occ = role == 0 ? i : (role == 1 ? j : k);
-        double* dst = (type == 0 ? IntX : IntY) + x + (long)nv * occ;
-        *dst += w * red[t];
+    } else {
+      const int ct = g - 2 * ntile;
+      const double* src = pm + (long)ct * nt2 * NPART + 48;
+      double s = 0.0;
+      for (long u = t; u < nt2; u += RED_T) s += src[u * NPART];
+      red[t] = s;
+      __syncthreads();
+      for (int h = RED_T / 2; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
       }
+      if (t == 0) eacc[ct] += w * red[0];
       __syncthreads();
     }
-  } else {
-    const int ct = g - 2 * ntile;
-    const double* src = part + (long)ct * nt2 * NPART + 48;
-    double s = 0.0;
-    for (long u = t; u < nt2; u += RED_T) s += src[u * NPART];
-    red[t] = s;
-    __syncthreads();
-    for (int h = RED_T / 2; h > 0; h >>= 1) {
-      if (t < h) red[t] += red[t + h];
-      __syncthreads();
-    }
-    if (t == 0) eacc[ct] += w * red[0];
   }
 }
 
@@ -366,43 +381,52 @@ void pack_amps(GemmEngine& g, int v, int o, const double* T, PackedAmps& P) {
   P.Ts.make(g, T, (long)o * o * v, v, 1, vv, v, v);
 }
 
-// K[a,b,c] (FP32) for triplet (i,j,k) from packed planes: the same twelve products
-// as build_K, each class with the pair of K indices carried by the integral (or
-// amplitude pair) operand as M (v^2 rows) and the single index as N (v), so
-// all three launches have the shape v^2 x v x (2v + 2o).  K is written in 8x8x8
-// boxes (the combine's unit), one contiguous 512-element run per box.
-void build_K_packed(GemmEngine& g, int v, int o, int i, int j, int k, const PackedAmps& T, const PackedInts& I,
-                    float* const K[3]) {
+// K[a,b,c] (FP32) for the triplets (i0 + z, j, k), z < nb, from packed planes: the
+// same twelve products as build_K, each class with the pair of K indices carried
+// by the integral (or amplitude pair) operand as M (v^2 rows) and the single index
+// as N (v), so all three launches have the shape v^2 x v x (2v + 2o), batched over
+// z (every operand offset is affine in i).  K is written in 8x8x8 boxes (the
+// combine's unit), one contiguous 512-element run per box; member z's parts start
+// kz elements after member z-1's.
+void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, const PackedAmps& T,
+                    const PackedInts& I, float* const K[3], long kz) {
   const long vv = (long)v * v;
   auto rowsW = [&](const Planes& P, int m) { return m * vv * P.ld; };           // block m of a v^2-row family
   auto rowsS = [&](const Planes& P, int p, int q) { return (p + (long)o * q) * v * P.ld; };   // v-row block
-  auto seg = [&](PackedSeg& s, const Planes& A, long aoff, const Planes& B, long boff) {
-    s.Ah = A.hi + aoff; s.Al = A.lo + aoff; s.lda = A.ld;
-    s.Bh = B.hi + boff; s.Bl = B.lo + boff; s.ldb = B.ld;
+  // operand offsets as functions of i; the batch advances them by their change per unit i
+  auto seg = [&](PackedSeg& s, const Planes& A, auto aoff, const Planes& B, auto boff) {
+    const long a0 = aoff(i0), b0 = boff(i0);
+    s.Ah = A.hi + a0; s.Al = A.lo + a0; s.lda = A.ld; s.a_z = aoff(i0 + 1) - a0;
+    s.Bh = B.hi + b0; s.Bl = B.lo + b0; s.ldb = B.ld; s.b_z = boff(i0 + 1) - b0;
     s.K = (int)A.ld;
   };
   PackedOp op;
   op.M = (int)vv; op.N = v; op.nseg = 4; op.Cf = K[0];
   op.algorithmic_k = 2LL * v + 2LL * o;
-  // class a: m = (b,c), n = a; K[a,b,c] at a + v*(b + v c)
-  seg(op.seg[0], I.W, rowsW(I.W, k), T.Ts, rowsS(T.Ts, i, j));      // 1: T^{ij}[a,d] W_k[b,c,d]
-  seg(op.seg[1], I.Wt, rowsW(I.Wt, j), T.Ts, rowsS(T.Ts, i, k));    // 3: T^{ik}[a,d] W_j[c,b,d]
-  seg(op.seg[2], T.T3, rowsW(T.T3, j), I.Hn, rowsS(I.Hn, k, i));    // 11: -H[l,a,k,i] T[b,c,j,l]
-  seg(op.seg[3], T.T4, rowsW(T.T4, k), I.Hn, rowsS(I.Hn, j, i));    // 12: -H[l,a,j,i] T[b,c,l,k]
+  op.batch = nb; op.c_z = kz;
+  // class a: m = (b,c), n = a
+  seg(op.seg[0], I.W, [&](int) { return rowsW(I.W, k); }, T.Ts, [&](int i) { return rowsS(T.Ts, i, j); });
+  seg(op.seg[1], I.Wt, [&](int) { return rowsW(I.Wt, j); }, T.Ts, [&](int i) { return rowsS(T.Ts, i, k); });
+  seg(op.seg[2], T.T3, [&](int) { return rowsW(T.T3, j); }, I.Hn, [&](int i) { return rowsS(I.Hn, k, i); });
+  seg(op.seg[3], T.T4, [&](int) { return rowsW(T.T4, k); }, I.Hn, [&](int i) { return rowsS(I.Hn, j, i); });
+  // terms 1: T^{ij}[a,d] W_k[b,c,d], 3: T^{ik}[a,d] W_j[c,b,d], 11: -H[l,a,k,i] T[b,c,j,l],
+  //       12: -H[l,a,j,i] T[b,c,l,k]
   op.box_class = 1; op.box_v = v; op.beta = 0.0;
   g.run_bf16x3_packed(op);
-  // class b: m = (a,c), n = b; K at a + v^2 c + v b
-  seg(op.seg[0], I.W, rowsW(I.W, k), T.Ts, rowsS(T.Ts, j, i));      // 2: W_k[a,c,d] T^{ji}[b,d]
-  seg(op.seg[1], I.Wt, rowsW(I.Wt, i), T.Ts, rowsS(T.Ts, j, k));    // 5: W_i[c,a,d] T^{jk}[b,d]
-  seg(op.seg[2], T.T3, rowsW(T.T3, i), I.Hn, rowsS(I.Hn, k, j));    // 9: T[a,c,i,l] (-H[l,b,k,j])
-  seg(op.seg[3], T.T4, rowsW(T.T4, k), I.Hn, rowsS(I.Hn, i, j));    // 10: T[a,c,l,k] (-H[l,b,i,j])
+  // class b: m = (a,c), n = b; terms 2: W_k[a,c,d] T^{ji}[b,d], 5: W_i[c,a,d] T^{jk}[b,d],
+  //       9: T[a,c,i,l] (-H[l,b,k,j]), 10: T[a,c,l,k] (-H[l,b,i,j])
+  seg(op.seg[0], I.W, [&](int) { return rowsW(I.W, k); }, T.Ts, [&](int i) { return rowsS(T.Ts, j, i); });
+  seg(op.seg[1], I.Wt, [&](int i) { return rowsW(I.Wt, i); }, T.Ts, [&](int) { return rowsS(T.Ts, j, k); });
+  seg(op.seg[2], T.T3, [&](int i) { return rowsW(T.T3, i); }, I.Hn, [&](int) { return rowsS(I.Hn, k, j); });
+  seg(op.seg[3], T.T4, [&](int) { return rowsW(T.T4, k); }, I.Hn, [&](int i) { return rowsS(I.Hn, i, j); });
   op.box_class = 2; op.Cf = K[1];
   g.run_bf16x3_packed(op);
-  // class c: m = (a,b), n = c; K at (a + v b) + v^2 c
-  seg(op.seg[0], I.W, rowsW(I.W, j), T.Ts, rowsS(T.Ts, k, i));      // 4: W_j[a,b,d] T^{ki}[c,d]
-  seg(op.seg[1], I.Wt, rowsW(I.Wt, i), T.Ts, rowsS(T.Ts, k, j));    // 6: W_i[b,a,d] T^{kj}[c,d]
-  seg(op.seg[2], T.T3, rowsW(T.T3, i), I.Hn, rowsS(I.Hn, j, k));    // 7: T[a,b,i,l] (-H[l,c,j,k])
-  seg(op.seg[3], T.T4, rowsW(T.T4, j), I.Hn, rowsS(I.Hn, i, k));    // 8: T[a,b,l,j] (-H[l,c,i,k])
+  // class c: m = (a,b), n = c; terms 4: W_j[a,b,d] T^{ki}[c,d], 6: W_i[b,a,d] T^{kj}[c,d],
+  //       7: T[a,b,i,l] (-H[l,c,j,k]), 8: T[a,b,l,j] (-H[l,c,i,k])
+  seg(op.seg[0], I.W, [&](int) { return rowsW(I.W, j); }, T.Ts, [&](int i) { return rowsS(T.Ts, k, i); });
+  seg(op.seg[1], I.Wt, [&](int i) { return rowsW(I.Wt, i); }, T.Ts, [&](int) { return rowsS(T.Ts, k, j); });
+  seg(op.seg[2], T.T3, [&](int i) { return rowsW(T.T3, i); }, I.Hn, [&](int) { return rowsS(I.Hn, j, k); });
+  seg(op.seg[3], T.T4, [&](int) { return rowsW(T.T4, j); }, I.Hn, [&](int i) { return rowsS(I.Hn, i, k); });
   op.box_class = 3; op.Cf = K[2];
   g.run_bf16x3_packed(op);
 }
@@ -507,11 +531,15 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   const PackedInts& PIU = (lambda && !c.hermitian) ? PIL : PI;
   const int ntile = (v + TT - 1) / TT;
   const long nblocks = (long)ntile * ntile * ntile;
+  // the packed path evaluates blocks of up to BMAX triplets (i0 + z, j, k) per launch
+  const int BMAX = packed ? 8 : 1;
   // the packed path stores K in whole 8x8x8 boxes (ntile^3 of them, padding zeroed below)
-  const size_t kbytes = packed ? sizeof(float) * nblocks * TT * TT * TT : sizeof(double) * vvv;
+  const long kmem = packed ? nblocks * TT * TT * TT : vvv;   // elements of one part of one triplet
+  const size_t kbytes = (packed ? sizeof(float) : sizeof(double)) * kmem * BMAX;   // one part, all members
+  const long pz = nblocks * NPART;
   DevBuf& K = W[23];
   DevBuf& KU = W[24];
-  DevBuf part(sizeof(double) * nblocks * NPART);
+  DevBuf part(sizeof(double) * pz * BMAX);
   K.ensure(3 * kbytes);
   if (lambda) KU.ensure(3 * kbytes);
   if (packed) {
@@ -579,6 +607,8 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   cf.eo = ca.eo; cf.ev = ca.ev; cf.W = ca.W; cf.Y = ca.Y; cf.part = ca.part;
   ca.boxed = 0;
   cf.boxed = 1;
+  ca.kz = kmem; cf.kz = kmem;
+  ca.pz = pz; cf.pz = pz;
   std::copy(&ca.nblk[0], &ca.nblk[0] + 4, &cf.nblk[0]);
   std::copy(&ca.blk[0][0][0], &ca.blk[0][0][0] + 4 * 6 * 7, &cf.blk[0][0][0]);
   for (int q = 0; q < 3; ++q) {
@@ -586,31 +616,75 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
     cf.K[q] = Kf[q]; cf.KU[q] = lambda ? KUf[q] : nullptr;
   }
   const unsigned cgrid = ntri;   // one workgroup per sorted tile triple
-  // loop order of cc_triples.jl:61-71
-  long tidx = -1;
+  // one block of triplets (i0 + z, j, k), z < nb: products, combine, and (when reduce) the
+  // ordered accumulation into IntX, IntY and the energy
+  auto run_block = [&](int i0, int nb, int j, int k, bool reduce) {
+    if (packed) {
+      build_K_packed(ops.gemm, v, o, i0, nb, j, k, PT, PI, Kf, kmem);
+      if (lambda) build_K_packed(ops.gemm, v, o, i0, nb, j, k, PU, PIU, KUf, kmem);
+      cf.i = i0; cf.j = j; cf.k = k;
+      hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, cf);
+    } else {
+      build_K(ops.gemm, pr, v, o, i0, j, k, dT2.d(), IT, Kd);
+      if (lambda) build_K(ops.gemm, pr, v, o, i0, j, k, dU2.d(), IL, KUd);
+      ca.i = i0; ca.j = j; ca.k = k;
+      hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid, 1u), dim3(256), 0, s, ca);
+    }
+    if (reduce)
+      hipLaunchKernelGGL(triples_reduce_kernel, dim3(3 * ntile), dim3(RED_T), 0, s, v, ntile, i0, nb, j, k,
+                         part.d(), pz, IntX.d(), IntY.d(), eacc.d());
+    ECG_HIP(hipGetLastError());
+  };
+  // block size of the packed path: chosen by measurement on this context (and kept), or
+  // from the tile count of one triplet's products for short ranges
+  const long ntrip_all = (long)o * (o + 1) * (o + 2) / 6 - o;
+  const long nrange = count < 0 ? ntrip_all : std::min<long>(count, std::max<long>(0, ntrip_all - first));
+  int nbatch = 1;
+  if (packed) {
+    int& tuned = c.triples_batch[lambda ? 1 : 0];
+    const long tiles = (((long)v * v + 255) / 256) * ((v + 127) / 128);
+    nbatch = (int)std::max<long>(1, std::min<long>(BMAX, 1024 / std::max<long>(1, tiles)));
+    if (tuned > 0) {
+      nbatch = tuned;
+    } else if (nrange >= 64 && o >= 3) {
+      // time blocks of 1, 2, 4, 8 triplets (i, o-1, o-1) on the real operands; nothing is
+      // accumulated (the reduce is skipped), the main loop overwrites K and the records
+      hipEvent_t e0, e1;
+      ECG_HIP(hipEventCreate(&e0));
+      ECG_HIP(hipEventCreate(&e1));
+      const int avail = o - 1;   // i < o - 1 for (j, k) = (o-1, o-1)
+      double best = HUGE_VAL;
+      for (int nb = 1; nb <= std::min(BMAX, avail); nb *= 2) {
+        run_block(0, nb, o - 1, o - 1, false);   // warm-up of this shape
+        ECG_HIP(hipEventRecord(e0, s));
+        for (int rep = 0; rep < 2; ++rep) run_block(0, nb, o - 1, o - 1, false);
+        ECG_HIP(hipEventRecord(e1, s));
+        ECG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        ECG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / (2.0 * nb);
+        if (per < 0.97 * best) { best = per; nbatch = nb; }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      tuned = nbatch;
+    }
+  }
+  c.triples_last_batch = nbatch;
+  // loop order of cc_triples.jl:61-71 (k, j <= k, i <= j, i = j = k skipped): the triplets of
+  // one (j, k) are consecutive in i, so a block is a run of consecutive triplets
+  long g0 = 0;
   const long last = count < 0 ? (1L << 62) : first + count;
   for (int k = 0; k < o; ++k)
-    for (int j = 0; j <= k; ++j)
-      for (int i = 0; i <= j; ++i) {
-        if (i == j && j == k) continue;
-        ++tidx;
-        if (tidx < first || tidx >= last) continue;
-        double w = (i < j && j < k) ? 2.0 : 1.0;
-        if (packed) {
-          build_K_packed(ops.gemm, v, o, i, j, k, PT, PI, Kf);
-          if (lambda) build_K_packed(ops.gemm, v, o, i, j, k, PU, PIU, KUf);
-          cf.i = i; cf.j = j; cf.k = k;
-          hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid), dim3(256), 0, s, cf);
-        } else {
-          build_K(ops.gemm, pr, v, o, i, j, k, dT2.d(), IT, Kd);
-          if (lambda) build_K(ops.gemm, pr, v, o, i, j, k, dU2.d(), IL, KUd);
-          ca.i = i; ca.j = j; ca.k = k;
-          hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid), dim3(256), 0, s, ca);
-        }
-        hipLaunchKernelGGL(triples_reduce_kernel, dim3(3 * ntile), dim3(RED_T), 0, s, v, ntile, i, j, k, w,
-                           part.d(), IntX.d(), IntY.d(), eacc.d());
-        ECG_HIP(hipGetLastError());
+    for (int j = 0; j <= k; ++j) {
+      const long ni = (j == k) ? j : j + 1;
+      const long lo = std::max<long>(g0, first), hi = std::min<long>(g0 + ni, last);
+      for (long t0 = lo; t0 < hi; t0 += nbatch) {
+        const int nb = (int)std::min<long>(nbatch, hi - t0);
+        run_block((int)(t0 - g0), nb, j, k, true);
       }
+      g0 += ni;
+    }
   // E(T) = E[T] + T1.IntX + f_ov.IntY  (Lambda: 1/2 U1.IntX)
   std::vector<double> hx(n1), hy(n1), hf((size_t)c.n * c.n), he(ntile);
   ECG_HIP(hipMemcpyAsync(hx.data(), IntX.p, IntX.bytes, hipMemcpyDeviceToHost, s));

@@ -260,10 +260,10 @@ class Context:
         self._chk(self._lib.ecg_profile(self._h, int(enable)))
 
     def stats(self):
-        s = np.zeros(7)
+        s = np.zeros(8)
         self._chk(self._lib.ecg_stats(self._h, _ptr(s)))
         return dict(residual_ms=s[0], ladder_ms=s[1], gemm_ms=s[2], gemm_launches=int(s[3]), gemm_flops=s[4],
-                    fp64_iterations=int(s[5]), gemm_bytes=s[6])
+                    fp64_iterations=int(s[5]), gemm_bytes=s[6], triples_block=int(s[7]))
 
 
 def cc_solve_batch(contexts, method=ECG_CCSD, precision=ECG_FP64, opts=None, max_threads=0):

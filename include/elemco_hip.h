@@ -151,8 +151,11 @@ int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* 
  * every MFMA GEMM launch on the library stream and zeroes the counters.
  * stats = [last residual ms, last ladder ms, timed GEMM ms, timed GEMM launches,
  *          issued GEMM flops, FP64 iterations of the last ecg_cc_solve,
- *          algorithmic GEMM bytes (operands once, C written once or read and written)]
- * (7 doubles) */
+ *          algorithmic GEMM bytes (operands once, C written once or read and written),
+ *          triplets per block of the last (T) call]
+ * (8 doubles).  The BF16x3 (T) evaluates blocks of consecutive triplets (same j, k)
+ * per launch; the block size (1, 2, 4 or 8) is measured on the context's first (T)
+ * call of at least 64 triplets and kept; results do not depend on it. */
 int ecg_profile(ecg_ctx* ctx, int enable);
 int ecg_stats(ecg_ctx* ctx, double* stats);
 

@@ -92,3 +92,24 @@ def test_mixed_triples_against_oracle(cfg, prec):
     lt64, ltb64 = EC.ctx.pert_triples(T1, T2, U1, U2)
     lt, ltb = EC.ctx.pert_triples(T1, T2, U1, U2, precision=prec)
     assert abs(lt - lt64) < tol * abs(lt64) and abs(ltb - ltb64) < tol * abs(ltb64), (lt, lt64, ltb, ltb64)
+
+
+def test_triples_blocks_are_tuned_and_do_not_change_results():
+    """BF16x3 (T) in blocks of consecutive triplets: the first call of >= 64 triplets measures
+    the block size (1, 2, 4 or 8) and keeps it.  Members are reduced in triplet order, so a
+    second call gives bitwise the same energies; disjoint ranges add up to the whole (1e-12
+    relative); the result is within the BF16 hi/lo bar (4e-5) of FP64."""
+    cfg = (8, 30, 60, 3)
+    EC = E.ECInfo.synthetic(*cfg, scale=0.7 * conditioned_scale(*cfg[:3]))
+    T1, T2, _ = EC.ctx.mp2()
+    et64, etb64 = EC.ctx.pert_triples(T1, T2)
+    e1 = EC.ctx.pert_triples(T1, T2, precision=E.ECG_BF16X3)
+    nb = EC.ctx.stats()["triples_block"]
+    assert nb in (1, 2, 4, 8)
+    e2 = EC.ctx.pert_triples(T1, T2, precision=E.ECG_BF16X3)
+    assert e1 == e2
+    assert abs(e1[0] - et64) < 4e-5 * abs(et64) and abs(e1[1] - etb64) < 4e-5 * abs(etb64)
+    n = 8 * 9 * 10 // 6 - 8
+    a = EC.ctx.pert_triples_range(T1, T2, 0, 37, precision=E.ECG_BF16X3)
+    b = EC.ctx.pert_triples_range(T1, T2, 37, n - 37, precision=E.ECG_BF16X3)
+    assert abs(a[1] + b[1] - e1[1]) < 1e-12 * abs(e1[1])
