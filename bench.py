@@ -194,6 +194,18 @@ def gradient_check(E, o, v, q, seed, meth, ndir=20, step=1e-4, thr=1e-20):
     U1, U2, lit, lconv = ctx.lambda_solve(T1, T2, meth, opts)
     out["lambda_solve"] = {"iterations": lit, "converged": lconv, "seconds": round(time.perf_counter() - t0, 2)}
     log(f"  lambda solve: it={lit} conv={lconv}")
+    # Lambda iterations per precision at the default threshold (1e-10): time and distance
+    # to the FP64 multipliers
+    lam = {}
+    for name, prec in (("fp64", E.ECG_FP64), ("bf16x3", E.ECG_BF16X3)):
+        t0 = time.perf_counter()
+        V1, V2, lit2, lconv2 = ctx.lambda_solve(T1, T2, meth, E.default_opts(), precision=prec)
+        lam[name] = {"iterations": lit2, "converged": lconv2, "fp64_iterations": ctx.stats()["fp64_iterations"],
+                     "seconds": round(time.perf_counter() - t0, 2),
+                     "max_abs_dU_vs_tight_fp64": float(max(np.max(np.abs(V1 - U1)), np.max(np.abs(V2 - U2))))}
+        log(f"  lambda solve {name} (thr 1e-10): it={lit2} conv={lconv2}")
+    lam["bf16x3"]["speedup_vs_fp64"] = round(lam["fp64"]["seconds"] / lam["bf16x3"]["seconds"], 3)
+    out["lambda_solve_thr1e-10"] = lam
     ctx.energy_grad(T1, T2, U1, U2, meth)                       # warm-up (allocations)
     t0 = time.perf_counter()
     gh, gb = ctx.energy_grad(T1, T2, U1, U2, meth)

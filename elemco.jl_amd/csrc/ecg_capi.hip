@@ -2,6 +2,7 @@
 #include "../../include/elemco_hip.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 #include <cstring>
 #include <memory>
@@ -186,7 +187,7 @@ int ecg_cc_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts*
     c.state_init(nullptr, nullptr, o);
     double e[6];
     int conv = 0;
-    int it = c.solve(method, to_prec(precision), o, e, &conv, nullptr);
+    int it = c.solve(method, to_prec(precision), o, e, &conv, std::getenv("ECG_ITERATION_LOG") ? stderr : nullptr);
     if (T1) d2h(T1, c.T1.p, sizeof(double) * c.nvo(), c.stream);
     if (T2) d2h(T2, c.T2.p, sizeof(double) * c.nvvoo(), c.stream);
     ECG_HIP(hipStreamSynchronize(c.stream));
@@ -276,13 +277,13 @@ int ecg_lambda_residual(ecg_ctx* ctx, int method, int precision, const double* T
   });
 }
 
-int ecg_lambda_solve(ecg_ctx* ctx, int method, const ecg_solver_opts* opts, const double* T1, const double* T2,
-                     double* U1, double* U2, int* iterations, int* converged) {
+int ecg_lambda_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, const double* T1,
+                     const double* T2, double* U1, double* U2, int* iterations, int* converged) {
   return guarded(ctx, [&](ecg::Context& c) {
     check_method(method);
     ECG_CHECK(T1 && T2 && U1 && U2, "NULL pointer");
     int it = 0, conv = 0;
-    ecg::lambda_solve(c, method, to_opts(opts), T1, T2, U1, U2, &it, &conv);
+    ecg::lambda_solve(c, method, to_prec(precision), to_opts(opts), T1, T2, U1, U2, &it, &conv);
     if (iterations) *iterations = it;
     if (converged) *converged = conv;
   });

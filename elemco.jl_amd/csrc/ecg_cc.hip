@@ -815,7 +815,12 @@ void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info,
 // norm drops below 100*thr, and whenever it stalls (less than 20% below the
 // smallest norm since the last refresh: the low-precision rounding floor).
 // Convergence is declared only on an exact (FP64) iteration, so the fixed
-// point is the FP64 solution.
+// point is the FP64 solution.  The corrected residual has a floor set by the
+// rounding of the low-precision products (their error does not shrink with
+// T - T_ref); when the first low-precision iteration after each of two refreshes
+// in a row already stalls, the floor is reached and the remaining iterations are
+// plain FP64 ones with a fresh DIIS history (needed only for thresholds below
+// that floor).
 int Context::solve(int method, Prec pr, const SolverOpts& o, double* energies, int* converged, FILE* log) {
   diis_reset(o.maxdiis, o.resthr);
   iter = 0;
@@ -823,14 +828,16 @@ int Context::solve(int method, Prec pr, const SolverOpts& o, double* energies, i
   double info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int conv = 0, it = 0;
   const bool mixed = (pr != Prec::F64);
-  bool refresh_next = false;
+  bool refresh_next = false, fp64_only = false, after_refresh = false;
+  int quick_stalls = 0;
   double best = HUGE_VAL;
   fp64_iterations = 0;
   for (it = 1; it <= o.maxit; ++it) {
     const bool refresh =
-        mixed && (refresh_next || (o.refine_every > 0 && (it % o.refine_every) == 0));
-    const bool exact = !mixed || refresh;
-    state_step(method, pr, o, info, refresh);
+        mixed && !fp64_only && (refresh_next || (o.refine_every > 0 && (it % o.refine_every) == 0));
+    const bool exact = !mixed || refresh || fp64_only;
+    if (fp64_only) state_step(method, Prec::F64, o, info, false);
+    else state_step(method, pr, o, info, refresh);
     if (exact) fp64_iterations++;
     if (log) {
       std::fprintf(log, "%4d %12.8f %16.10f %12.3e %12.3e %s\n", it, info[6], info[0], info[4], info[5],
@@ -841,9 +848,17 @@ int Context::solve(int method, Prec pr, const SolverOpts& o, double* energies, i
       conv = 1;
       break;
     }
-    if (mixed) {
+    if (mixed && !fp64_only) {
       if (refresh) best = HUGE_VAL;
       const bool stalled = info[5] > 0.8 * best;
+      if (!refresh && after_refresh) {
+        quick_stalls = stalled ? quick_stalls + 1 : 0;
+        fp64_only = quick_stalls >= 2;
+        // the DIIS history holds corrected residuals that do not match the exact ones
+        // (DIIS keeps its largest-coefficient vectors, so they would persist): restart it
+        if (fp64_only) diis_reset(o.maxdiis, o.resthr);
+      }
+      after_refresh = refresh;
       best = std::min(best, info[5]);
       refresh_next = info[5] < 100.0 * o.thr || stalled;
     }

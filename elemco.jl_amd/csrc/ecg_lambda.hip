@@ -13,6 +13,8 @@
 #include "ecg_lambda.h"
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace ecg {
@@ -348,9 +350,16 @@ void one_rdm_host(Context& c, const double* T1h, const double* T2h, const double
   if (dD1h) gather_nn(c, D.doo, D.dov, D.dvo, D.dvv, dD1h);
 }
 
-// lm_cc_iterations! closed-shell (cc_lagrange.jl:1155-1223)
-void lambda_solve(Context& c, int method, const SolverOpts& o, const double* T1h, const double* T2h, double* U1h,
-                  double* U2h, int* iterations, int* converged) {
+// lm_cc_iterations! closed-shell (cc_lagrange.jl:1155-1223).  pr != F64: defect
+// correction as in Context::solve -- iterations use R_low(U) + delta with delta =
+// R_64 - R_low refreshed at the current U every refine_every-th iteration, when
+// the corrected norm drops below 100*thr and when it stalls; convergence is
+// declared only on such an exact iteration; after two refreshes in a row whose
+// next low-precision iteration already stalls (the rounding floor of the
+// corrected residual), the remaining iterations are plain FP64 ones with a fresh
+// DIIS history.
+void lambda_solve(Context& c, int method, Prec pr, const SolverOpts& o, const double* T1h, const double* T2h,
+                  double* U1h, double* U2h, int* iterations, int* converged) {
   LambdaFixed F;
   prepare(c, T1h, T2h, F);
   const long n1 = c.nvo(), n2 = c.nvvoo(), v = c.nv, no = c.no;
@@ -386,8 +395,33 @@ void lambda_solve(Context& c, int method, const SolverOpts& o, const double* T1h
     ops.arena.reset();
     return s;
   };
+  const bool mixed = (pr != Prec::F64);
+  FILE* log = std::getenv("ECG_ITERATION_LOG") ? stderr : nullptr;   // per-iteration lines, opt-in
+  DevBuf D1, D2;   // FP64 correction of the low-precision residual
+  bool have_delta = false, refresh_next = false, fp64_only = false, after_refresh = false;
+  int quick_stalls = 0;
+  double best = HUGE_VAL;
+  int exact_count = 0;
   for (it = 1; it <= o.maxit; ++it) {
-    residual(c, F, method, Prec::F64, U1.d(), U2.d(), R1.d(), R2.d());
+    const bool refresh =
+        mixed && !fp64_only && (refresh_next || (o.refine_every > 0 && (it % o.refine_every) == 0));
+    const bool exact = !mixed || refresh || fp64_only;
+    residual(c, F, method, fp64_only ? Prec::F64 : pr, U1.d(), U2.d(), R1.d(), R2.d());
+    if (mixed && !fp64_only) {
+      if (refresh) {
+        D1.ensure(sizeof(double) * n1);
+        D2.ensure(sizeof(double) * n2);
+        residual(c, F, method, Prec::F64, U1.d(), U2.d(), D1.d(), D2.d());
+        ops.axpby(n1, -1.0, R1.d(), 1.0, D1.d());
+        ops.axpby(n2, -1.0, R2.d(), 1.0, D2.d());
+        have_delta = true;
+      }
+      if (have_delta) {
+        ops.axpby(n1, 1.0, D1.d(), 1.0, R1.d());
+        ops.axpby(n2, 1.0, D2.d(), 1.0, R2.d());
+      }
+    }
+    if (exact) exact_count++;
     // update_doubles!/update_singles! with the CC denominators and shifts
     c.update(U1.d(), U2.d(), R1.d(), R2.d(), o.shifts, o.shiftp);
     // calc_contra_doubles_norm (cctools.jl:487-490): sum (2R + R_ba) R / 3; singles: R.R / 2
@@ -397,11 +431,30 @@ void lambda_solve(Context& c, int method, const SolverOpts& o, const double* T1h
     double prev = ltnorm;
     ltnorm = corr_norm();
     double NormR = NormR1 + NormR2;
-    if (NormR < o.thr && std::fabs(ltnorm - prev) < thren) {
+    if (log) {
+      std::fprintf(log, "%4d %16.10f %12.3e %12.3e %s\n", it, ltnorm, ltnorm - prev, NormR, exact ? "fp64" : "low");
+      std::fflush(log);
+    }
+    if (exact && NormR < o.thr && std::fabs(ltnorm - prev) < thren) {
       conv = 1;
       break;
     }
+    if (mixed && !fp64_only) {
+      if (refresh) best = HUGE_VAL;
+      const bool stalled = NormR > 0.8 * best;
+      if (!refresh && after_refresh) {
+        quick_stalls = stalled ? quick_stalls + 1 : 0;
+        fp64_only = quick_stalls >= 2;
+        // the DIIS history holds corrected residuals that do not match the exact ones
+        // (DIIS keeps its largest-coefficient vectors, so they would persist): restart it
+        if (fp64_only) c.diis_reset(o.maxdiis, o.resthr);
+      }
+      after_refresh = refresh;
+      best = std::min(best, NormR);
+      refresh_next = NormR < 100.0 * o.thr || stalled;
+    }
   }
+  c.fp64_iterations = exact_count;
   if (it > o.maxit) it = o.maxit;
   ECG_HIP(hipMemcpyAsync(U1h, U1.p, U1.bytes, hipMemcpyDeviceToHost, c.stream));
   ECG_HIP(hipMemcpyAsync(U2h, U2.p, U2.bytes, hipMemcpyDeviceToHost, c.stream));

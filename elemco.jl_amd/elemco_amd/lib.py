@@ -53,7 +53,7 @@ _SIGS = {
     "ecg_pert_triples_range": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_long,
                                _dp, _dp],
     "ecg_lambda_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
-    "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
+    "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
     "ecg_one_rdm": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_energy_grad": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_profile": [_vp, ctypes.c_int],
@@ -229,13 +229,13 @@ class Context:
                                                 _ptr(R1), _ptr(R2)))
         return R1, R2
 
-    def lambda_solve(self, T1, T2, method=ECG_CCSD, opts=None):
+    def lambda_solve(self, T1, T2, method=ECG_CCSD, opts=None, precision=ECG_FP64):
         opts = opts or default_opts()
         T1, T2 = self._t1(T1), self._t2(T2)
         U1, U2 = self.zeros1(), self.zeros2()
         it, conv = ctypes.c_int(), ctypes.c_int()
-        self._chk(self._lib.ecg_lambda_solve(self._h, method, ctypes.byref(opts), _ptr(T1), _ptr(T2), _ptr(U1),
-                                             _ptr(U2), ctypes.byref(it), ctypes.byref(conv)))
+        self._chk(self._lib.ecg_lambda_solve(self._h, method, precision, ctypes.byref(opts), _ptr(T1), _ptr(T2),
+                                             _ptr(U1), _ptr(U2), ctypes.byref(it), ctypes.byref(conv)))
         return U1, U2, it.value, bool(conv.value)
 
     def one_rdm(self, T1, T2, U1, U2):

@@ -25,7 +25,7 @@ one-electron integrals.
 module ElemCoAMD
 
 export ECGContext, calc_cc_resid, calc_ccsd_vector_times_Jacobian, calc_1RDM, calc_pertT, calc_cc, calc_lm_cc,
-       SolverOpts, default_opts
+       energy_grad, SolverOpts, default_opts, ECG_FP64, ECG_FP32, ECG_BF16X3
 
 const LIB = get(ENV, "ELEMCO_HIP_LIB", joinpath(@__DIR__, "..", "libelemco_hip.so"))
 
@@ -33,6 +33,7 @@ const ECG_CCSD = Cint(0)
 const ECG_DCSD = Cint(1)
 const ECG_FP64 = Cint(0)
 const ECG_FP32 = Cint(1)
+const ECG_BF16X3 = Cint(2)
 
 "Mirror of `ecg_solver_opts` (defaults of `src/infos/options.jl:128-221,272-282`)."
 mutable struct SolverOpts
@@ -171,14 +172,14 @@ function calc_cc(ctx::ECGContext, method::AbstractString; opts=default_opts(), p
 end
 
 "`calc_lm_cc(EC, method)` (src/cc/cc_lagrange.jl:1120-1223): Λ multipliers `(U1, U2)` at amplitudes T1, T2."
-function calc_lm_cc(ctx::ECGContext, T1, T2, method::AbstractString; opts=default_opts())
+function calc_lm_cc(ctx::ECGContext, T1, T2, method::AbstractString; opts=default_opts(), precision=ECG_FP64)
   m = lowercase(method) == "dcsd" ? ECG_DCSD : ECG_CCSD
   U1, U2 = _z1(ctx), _z2(ctx)
   it, conv = Ref{Cint}(0), Ref{Cint}(0)
   st = ccall((:ecg_lambda_solve, LIB), Cint,
-             (Ptr{Cvoid}, Cint, Ref{SolverOpts}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{Cint},
-              Ref{Cint}),
-             ctx.handle, m, opts, T1, T2, U1, U2, it, conv)
+             (Ptr{Cvoid}, Cint, Cint, Ref{SolverOpts}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+              Ref{Cint}, Ref{Cint}),
+             ctx.handle, m, precision, opts, T1, T2, U1, U2, it, conv)
   _check(ctx, st)
   conv[] == 0 && @warn "Lambda iterations did not converge in $(it[]) iterations"
   return U1, U2

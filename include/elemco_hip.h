@@ -127,8 +127,12 @@ int ecg_lambda_residual(ecg_ctx* ctx, int method, int precision, const double* T
                         const double* U1, const double* U2, double* R1, double* R2);
 
 /* Lambda iterations lm_cc_iterations! (src/cc/cc_lagrange.jl:1155-1223) at
- * converged amplitudes T1, T2; U1/U2 receive the multipliers. */
-int ecg_lambda_solve(ecg_ctx* ctx, int method, const ecg_solver_opts* opts, const double* T1,
+ * converged amplitudes T1, T2; U1/U2 receive the multipliers.  precision =
+ * ECG_FP32 or ECG_BF16X3: the defect-corrected iterations of ecg_cc_solve
+ * applied to the Lambda residual (low-precision products plus an FP64 correction
+ * refreshed every refine_every-th iteration, near convergence and on stalls;
+ * convergence only on an exact iteration), so the result is the FP64 solution. */
+int ecg_lambda_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_opts* opts, const double* T1,
                      const double* T2, double* U1, double* U2, int* iterations, int* converged);
 
 /* Gradient of the correlation energy (the backward of the CC energy functional;

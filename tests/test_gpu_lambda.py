@@ -86,3 +86,32 @@ def test_h2o_lambda_ccsd_t_reference_energy():
     from oracle.fcidump import read_fcidump
     eo = O.ccdriver(O.ECInfo.from_fcidump(read_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"))), "λccsd(t)")
     assert abs(e["ΛCCSD(T)c"] - eo["ΛCCSD(T)c"]) < 1e-9
+
+
+@pytest.mark.parametrize("method", [E.ECG_CCSD, E.ECG_DCSD])
+@pytest.mark.parametrize("prec", [E.ECG_FP32, E.ECG_BF16X3])
+def test_mixed_lambda_solve_converges_to_fp64(method, prec):
+    """Defect-corrected low-precision Lambda iterations end at the FP64 multipliers:
+    the FP64 Lambda residual at the mixed solution meets thr = 1e-16 (products
+    rounded to FP32 without correction leave ~1e-7 relative residual errors, i.e.
+    a squared norm far above it), the solution agrees with the FP64 solve to 1e-8
+    (both within ~1e-8 of the fixed point at this threshold), convergence comes
+    on an FP64 iteration, and fewer FP64 residuals than iterations are used.
+    Sizes as in the mixed CC solve test (BF16 hi/lo products are used only for
+    products with M N / (M + N) >= 256)."""
+    from elemco_amd.synthetic import conditioned_scale
+    no, nv, nq, seed = (6, 40, 90, 21) if prec == E.ECG_FP32 else (10, 90, 300, 5)
+    EC = E.ECInfo.synthetic(no, nv, nq, seed, scale=0.7 * conditioned_scale(no, nv, nq))
+    opts = E.default_opts(maxit=300, thr=1e-16)
+    _, T1, T2, _, conv = EC.ctx.cc_solve(method, E.ECG_FP64, E.default_opts(maxit=300, thr=1e-20))
+    assert conv
+    U1a, U2a, it64, c64 = EC.ctx.lambda_solve(T1, T2, method, opts)
+    U1b, U2b, itlo, clo = EC.ctx.lambda_solve(T1, T2, method, opts, precision=prec)
+    n64 = EC.ctx.stats()["fp64_iterations"]
+    assert c64 and clo
+    assert 1 <= n64 < itlo
+    # the contravariant residual norm of lm_cc_iterations!: (2 R + R_ba) . R / 3 + R1 . R1 / 2
+    R1, R2 = EC.ctx.lambda_residual(T1, T2, U1b, U2b, method)
+    norm = float(np.sum((2.0 * R2 + R2.transpose(1, 0, 2, 3)) * R2) / 3.0 + 0.5 * np.sum(R1 * R1))
+    assert norm < 1e-16, norm
+    assert float(np.max(np.abs(U1b - U1a))) < 1e-8 and float(np.max(np.abs(U2b - U2a))) < 1e-8
