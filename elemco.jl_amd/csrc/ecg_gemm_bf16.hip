@@ -119,6 +119,64 @@ __device__ __forceinline__ long box_off(int m, int n, int cls, int bv) {
 
 __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 3); }
 
+// Epilogue of a 256x128 tile: element (row, col) of MFMA tile (i,j) sits in lane
+// col + 16*(row/4), slot row%4 (TO: lane row + 16*(col/4), slot col%4).  Per
+// column group: all 16 offsets, then all loads of C, then all stores (a load
+// issued after a store to C could alias it, which would serialise every element).
+template <bool TO>
+__device__ __forceinline__ void x3_epilogue(const X3Params& p, const f4_t (&acc)[4][4], int bm, int bn, int wm,
+                                            int wn, int li, int lk, int kz, long zb) {
+  double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0) + zb * p.cbz;
+  float* Cf = p.Cf ? p.Cf + zb * p.cbz : nullptr;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    long off[16];
+    bool ok[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = bn + wn * 64 + j * 16 + (TO ? 4 * lk + r : li);
+        const int m = bm + wm * 64 + i * 16 + (TO ? li : 4 * lk + r);
+        const int e = 4 * i + r;
+        ok[e] = n < p.N && m < p.M;
+        long oe;
+        if (p.box_class) {
+          oe = box_off(m, n, p.box_class, p.box_v);
+        } else {
+          const long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
+          const long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
+          oe = cm + cn;
+        }
+        off[e] = ok[e] ? oe : 0;
+      }
+    const bool rd = p.beta != 0.0;
+    if (Cf) {
+      float cv[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? Cf[off[e]] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = 4 * i + r;
+          if (ok[e]) Cf[off[e]] = (float)(p.alpha * (double)acc[i][j][r] + (rd ? p.beta * (double)cv[e] : 0.0));
+        }
+    } else {
+      double cv[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? C[off[e]] : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = 4 * i + r;
+          if (ok[e]) C[off[e]] = p.alpha * (double)acc[i][j][r] + (rd ? p.beta * cv[e] : 0.0);
+        }
+    }
+  }
+}
+
 // TO (transposed output): the MFMAs take B as their first operand, so each
 // accumulator holds D^T -- lanes run along m and the four slots along n -- and
 // the epilogue stores stay contiguous when m is the unit-stride output index.
@@ -252,59 +310,164 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
     __syncthreads();
   }
 
-  // epilogue: element (row, col) of tile (i,j) sits in lane col + 16*(row/4), slot row%4
-  // (TO: lane row + 16*(col/4), slot col%4).  Per column group: all 16 offsets, then
-  // all loads of C, then all stores (a load issued after a store to C could alias it,
-  // which would serialise every element).
-  double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0) + zb * p.cbz;
-  float* Cf = p.Cf ? p.Cf + zb * p.cbz : nullptr;
+  x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
+}
+
+// Operand planes and row strides of the segment holding slab k0 (uniform; static
+// argument indices only, so the arguments stay in SGPRs).
+struct SlabSrc {
+  const u16 *ah, *al, *bh, *bl;
+  long lda, ldb;
+  int kl;
+};
+template <bool MS>
+__device__ __forceinline__ SlabSrc slab_src(const X3Params& p, int k0, long zb) {
+  SlabSrc r{p.Ah[0], p.Al[0], p.Bh[0], p.Bl[0], p.kpad, p.kpad, k0};
+  if constexpr (MS) {
+    r.lda = p.lda[0]; r.ldb = p.ldb[0];
+    long oa = p.az[0], ob = p.bz[0];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    long off[16];
-    bool ok[16];
+    for (int q = 1; q < MAXSEG; ++q)
+      if (q < p.nseg && k0 >= p.kb[q]) {
+        r.ah = p.Ah[q]; r.al = p.Al[q]; r.bh = p.Bh[q]; r.bl = p.Bl[q];
+        r.lda = p.lda[q]; r.ldb = p.ldb[q]; r.kl = k0 - p.kb[q];
+        oa = p.az[q]; ob = p.bz[q];
+      }
+    r.ah += zb * oa; r.al += zb * oa; r.bh += zb * ob; r.bl += zb * ob;
+  }
+  return r;
+}
+
+typedef __attribute__((address_space(3))) void* lds_vptr;
+
+// 16 bytes per lane from global memory into LDS at M0 + 16 * lane (lane = position in
+// the wave).  Issued as inline assembly: the compiler then neither tracks these loads
+// (the kernel waits for them with explicit vmcnt waits) nor assumes they may alias
+// every LDS read that follows (which made it wait for all loads in flight).  One
+// wait state separates the M0 write from the load that reads it.
+__device__ __forceinline__ void dma16(const void* src, lds_vptr dst) {
+  const unsigned m0 = (unsigned)(__UINTPTR_TYPE__)dst;
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory");
+}
+
+// The same product with the operand slabs moved HBM/L2 -> LDS by the load unit
+// (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at M0 + 16 l), three
+// LDS buffers in rotation: slab s+2 is in flight while slab s is multiplied, with
+// no register staging.  Each lane fetches the chunk that its LDS slot holds under
+// the XOR swizzle, so the LDS layout (and the MFMA reads) are those of
+// gemm_bf16x3_kernel.  The three buffers are separate LDS objects and the loop is
+// unrolled by three, so the compiler sees which buffer a DMA writes and which one
+// the MFMAs read (no conservative wait on the loads in flight).
+template <bool TO, bool MS>
+__global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
+  __shared__ u32x4_t A0[2][XM][4], A1[2][XM][4], A2[2][XM][4];   // [plane][row][16-byte chunk]
+  __shared__ u32x4_t B0[2][XN][4], B1[2][XN][4], B2[2][XN][4];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave index in an SGPR
+  const int wm = wave & 3, wn = wave >> 2;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
+  const int bm = tm * XM, bn = tn * XN;
+  const int kz = blockIdx.z;
+  const long zb = blockIdx.y;
+  const int kbeg = kz * p.kchunk;
+  const int kend = min(p.kpad, kbeg + p.kchunk);
+
+  // slab k0 -> one buffer: A is 32 blocks of 16 rows x 64 bytes (2 planes x 16), B 16 blocks;
+  // wave w moves blocks w, w+8, ... (4 of A, 2 of B: six 1 KB loads per wave)
+  const int lrow = lane >> 2, lslot = lane & 3;
+  auto issue = [&](u32x4_t (&As)[2][XM][4], u32x4_t (&Bs)[2][XN][4], int k0) {
+    const SlabSrc g = slab_src<MS>(p, k0, zb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = wave + 8 * j, pl = c >> 4, row = 16 * (c & 15) + lrow;
+      const int ch = lslot ^ ((row >> 1) & 3);
+      const u16* src = (pl ? g.al : g.ah) + (size_t)(bm + row) * g.lda + g.kl + 8 * ch;
+      dma16(src, (lds_vptr)&As[pl][16 * (c & 15)][0]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wave + 8 * j, pl = c >> 3, row = 16 * (c & 7) + lrow;
+      const int ch = lslot ^ ((row >> 1) & 3);
+      const u16* src = (pl ? g.bl : g.bh) + (size_t)(bn + row) * g.ldb + g.kl + 8 * ch;
+      dma16(src, (lds_vptr)&Bs[pl][16 * (c & 7)][0]);
+    }
+  };
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  const int li = lane & 15, lk = lane >> 4;
+  auto compute = [&](const u32x4_t (&As)[2][XM][4], const u32x4_t (&Bs)[2][XN][4]) {
+    bf16x8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = wm * 64 + i * 16 + li;
+      ah[i] = __builtin_bit_cast(bf16x8_t, As[0][r][swz(r, lk)]);
+      al[i] = __builtin_bit_cast(bf16x8_t, As[1][r][swz(r, lk)]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = wn * 64 + j * 16 + li;
+      bh[j] = __builtin_bit_cast(bf16x8_t, Bs[0][c][swz(c, lk)]);
+      bl[j] = __builtin_bit_cast(bf16x8_t, Bs[1][c][swz(c, lk)]);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = bn + wn * 64 + j * 16 + (TO ? 4 * lk + r : li);
-        const int m = bm + wm * 64 + i * 16 + (TO ? li : 4 * lk + r);
-        const int e = 4 * i + r;
-        ok[e] = n < p.N && m < p.M;
-        long oe;
-        if (p.box_class) {
-          oe = box_off(m, n, p.box_class, p.box_v);
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (TO) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
         } else {
-          const long cn = p.cplain ? (long)n * p.ldc_ws : split_off(n, p.c_nd0, p.c_ns0, p.c_ns1);
-          const long cm = p.cplain ? (long)m : split_off(m, p.c_md0, p.c_ms0, p.c_ms1);
-          oe = cm + cn;
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
-        off[e] = ok[e] ? oe : 0;
       }
-    const bool rd = p.beta != 0.0;
-    if (Cf) {
-      float cv[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? Cf[off[e]] : 0.f;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int e = 4 * i + r;
-          if (ok[e]) Cf[off[e]] = (float)(p.alpha * (double)acc[i][j][r] + (rd ? p.beta * (double)cv[e] : 0.0));
-        }
-    } else {
-      double cv[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) cv[e] = (rd && ok[e]) ? C[off[e]] : 0.0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int e = 4 * i + r;
-          if (ok[e]) C[off[e]] = p.alpha * (double)acc[i][j][r] + (rd ? p.beta * cv[e] : 0.0);
-        }
-    }
+  };
+  // slab k0 of this split, clamped to the first one past the end (such loads fill a
+  // buffer that is never multiplied)
+  auto slab = [&](int k0) { return k0 < kend ? k0 : kbeg; };
+  // this wave's loads of the slab issued before the most recent one have landed
+  auto wait_prev = []() { asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); };
+  // workgroup barrier without the memory fence of the HIP barrier (which waits for all
+  // loads, including the slab still in flight): the slab made visible by it is the one
+  // wait_prev covers, and every LDS read of the buffer reloaded next has completed (the
+  // MFMAs used it); the empty asm statements keep the compiler from moving LDS accesses
+  // across it
+  auto barrier = []() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+  if (kbeg < kend) {
+    issue(A0, B0, kbeg);
+    issue(A1, B1, slab(kbeg + XK));
+    wait_prev();
   }
+  barrier();
+  for (int k0 = kbeg; k0 < kend; k0 += 3 * XK) {
+    issue(A2, B2, slab(k0 + 2 * XK));
+    compute(A0, B0);
+    wait_prev();
+    barrier();
+    if (k0 + XK >= kend) break;
+    issue(A0, B0, slab(k0 + 3 * XK));
+    compute(A1, B1);
+    wait_prev();
+    barrier();
+    if (k0 + 2 * XK >= kend) break;
+    issue(A1, B1, slab(k0 + 4 * XK));
+    compute(A2, B2);
+    wait_prev();
+    barrier();
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
+  x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
 }
 
 }  // namespace
@@ -381,8 +544,12 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   }
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
-  hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0,
-                     stream, x);
+  if (bf16_dma)
+    hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512),
+                       0, stream, x);
+  else
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0,
+                       stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
   if (ksplit > 1) reduce_slabs(op, ksplit);
@@ -445,12 +612,15 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   ECG_CHECK(tiles < (1L << 31), "gemm: too many tiles");
   hipEvent_t e0 = timing_begin();
   // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
-  if (op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1))
-    hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), dim3((unsigned)tiles, (unsigned)op.batch, 1), dim3(512), 0,
-                       stream, x);
-  else
-    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, true>), dim3((unsigned)tiles, (unsigned)op.batch, 1), dim3(512), 0,
-                       stream, x);
+  const bool to = op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1);
+  const dim3 grid((unsigned)tiles, (unsigned)op.batch, 1);
+  if (bf16_dma) {
+    if (to) hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<true, true>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, true>), grid, dim3(512), 0, stream, x);
+  } else {
+    if (to) hipLaunchKernelGGL((gemm_bf16x3_kernel<true, true>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_bf16x3_kernel<false, true>), grid, dim3(512), 0, stream, x);
+  }
   ECG_HIP(hipGetLastError());
   timing_end(e0);
 }
