@@ -107,7 +107,9 @@ struct GemmEngine {
   // BF16 hi/lo products (ecg_gemm_bf16.hip): operands packed into zero-padded,
   // K-contiguous BF16 hi and lo planes in `pk`, 256x128 MFMA tiles
   DevBuf pk;
-  bool bf16_dma = true;   // BF16 hi/lo kernel with direct HBM -> LDS slab loads (else register staging)
+  // BF16 hi/lo GEMM kernel: 0 register-staged slabs (gemm_bf16x3_kernel), 1 slabs loaded
+  // straight into LDS (gemm_bf16x3_dma_kernel)
+  int bf16_kernel = 1;
   void run_bf16x3(const GemmOp& op);
   // Pack rows r < rows of src (row offset split as in GemmOp: (r % d0)*s0 + (r / d0)*s1,
   // element k at + k*sk, k < K) into hi/lo planes [r][kpad], kpad a multiple of 32 >= K.

@@ -544,12 +544,11 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   }
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
-  if (bf16_dma)
-    hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512),
-                       0, stream, x);
+  const dim3 grid((unsigned)tiles, 1, (unsigned)ksplit);
+  if (bf16_kernel == 1)
+    hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
   else
-    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), dim3((unsigned)tiles, 1, (unsigned)ksplit), dim3(512), 0,
-                       stream, x);
+    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), grid, dim3(512), 0, stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
   if (ksplit > 1) reduce_slabs(op, ksplit);
@@ -614,7 +613,7 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
   const bool to = op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1);
   const dim3 grid((unsigned)tiles, (unsigned)op.batch, 1);
-  if (bf16_dma) {
+  if (bf16_kernel == 1) {
     if (to) hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<true, true>), grid, dim3(512), 0, stream, x);
     else hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, true>), grid, dim3(512), 0, stream, x);
   } else {

@@ -113,6 +113,15 @@ double ecgt_bench_contract(int prec, int na, const long* adims, const char* la, 
   }
 }
 
+// Selects the BF16 hi/lo GEMM kernel of the hooks' engine (GemmEngine::bf16_kernel);
+// returns the previous choice.
+int ecgt_set_bf16_kernel(int k) {
+  Env& e = env();
+  const int prev = e.ops.gemm.bf16_kernel;
+  e.ops.gemm.bf16_kernel = k;
+  return prev;
+}
+
 // Lane-layout probe of the FP64 and FP32 16x16x4 MFMA builtins (64 lanes, in bounds).
 int ecgt_mfma_probe(const double* a, const double* b, double* out, const float* af, const float* bf, float* outf) {
   try {

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from gpu_helpers import gpu_contract, gpu_permute, requires_gpu
+from gpu_helpers import gpu_contract, gpu_permute, hooks, requires_gpu
 
 pytestmark = [pytest.mark.gpu, requires_gpu]
 
@@ -84,19 +84,34 @@ def test_contract_fp32_products(la, lb, lc):
     assert float(np.max(np.abs(got - ref))) < bound
 
 
-@pytest.mark.parametrize("la,lb,lc", [("akic", "cbkj", "abij"), ("klcd", "cdij", "klij"), ("ad", "dbij", "abij"),
-                                      ("aiQ", "kcQ", "akic")])
-def test_contract_bf16x3_products(la, lb, lc):
-    """BF16 hi/lo products (three BF16 MFMA terms, FP32 accumulation).
+# BF16 cases: M and N large enough for the BF16 path (M N / (M + N) >= 256), K from one
+# 32-wide slab (k = 19) over two (Q = 60) and four (Q = 97) to many with split K (917)
+BF16_CASES = [
+    (dict(a=131, b=131, i=7, j=7, k=19), "aik", "kbj", "abij"),
+    (dict(a=131, i=7, k=7, c=131, Q=60), "aiQ", "kcQ", "akic"),
+    (dict(a=131, i=7, k=7, c=131, Q=97), "aiQ", "kcQ", "akic"),
+    (dict(a=131, b=131, c=131, i=7, j=7, k=7), "akic", "cbkj", "abij"),
+]
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("case", range(len(BF16_CASES)))
+def test_contract_bf16x3_products(kernel, case):
+    """BF16 hi/lo products (three BF16 MFMA terms, FP32 accumulation), for each of the
+    engine's BF16 kernels (register-staged slabs; slabs loaded straight into LDS).
 
     Each operand is represented to 2^-17 relative and the dropped lo*lo term is
     below 2^-16 relative, so |error| <= (2^-14 + 4 sqrt(K) 2^-23) * (|A||B|);
     a lost hi*lo term (error ~2^-9 relative) would fail this bound."""
-    ext = SIZES["ragged"]
+    ext, la, lb, lc = BF16_CASES[case]
     rng = np.random.default_rng(19)
     A, B, C = _rand(la, ext, rng), _rand(lb, ext, rng), _rand(lc, ext, rng)
     ref = np.einsum(f"{la},{lb}->{lc}", A, B) + C
-    got = gpu_contract(1.0, A, la, B, lb, 1.0, C, lc, prec=2)
+    prev = hooks().ecgt_set_bf16_kernel(kernel)
+    try:
+        got = gpu_contract(1.0, A, la, B, lb, 1.0, C, lc, prec=2)
+    finally:
+        hooks().ecgt_set_bf16_kernel(prev)
     K = np.prod([ext[c] for c in la if c not in lc])
     scale = np.einsum(f"{la},{lb}->{lc}", np.abs(A), np.abs(B)).max()
     bound = (2.0**-14 + 4.0 * np.sqrt(K) * 2.0**-23) * scale
