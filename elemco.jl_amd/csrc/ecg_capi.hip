@@ -10,12 +10,20 @@
 #include <vector>
 
 #include "ecg_cc.h"
+#include "ecg_fcidump.h"
 #include "ecg_grad.h"
 #include "ecg_lambda.h"
 #include "ecg_triples.h"
 
 struct ecg_ctx {
   ecg::Context c;
+};
+
+struct ecg_fcidump {
+  ecg::Fcidump fd;
+  bool df_done = false;
+  double df_rtol = 0.0;
+  ecg::DfFactors df;
 };
 
 namespace {
@@ -104,6 +112,150 @@ int ecg_destroy(ecg_ctx* ctx) {
 const char* ecg_last_error(const ecg_ctx* ctx) {
   if (!ctx) return g_create_error.c_str();
   return ctx->c.last_error.c_str();
+}
+
+}  // extern "C"
+
+namespace {
+
+// host-side calls without a context: errors go to the thread's create-error slot
+template <class F>
+int host_guarded(F&& f) {
+  try {
+    return f();
+  } catch (const ecg::Error& e) {
+    g_create_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return -9;
+  }
+}
+
+int copy_text(const std::string& s, char* buf, int buflen) {
+  if (!buf || buflen < (int)s.size() + 1) {
+    g_create_error = "buffer too small";
+    return -2;
+  }
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return 0;
+}
+
+const ecg::DfFactors& fcidump_df(ecg_fcidump* f, double rtol) {
+  if (rtol <= 0.0) rtol = 1e-14;
+  if (!f->df_done || f->df_rtol != rtol) {
+    f->df = ecg::df_factorise(f->fd.eri, f->fd.norb, !f->fd.simtra(), rtol);
+    f->df_done = true;
+    f->df_rtol = rtol;
+  }
+  return f->df;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ecg_fcidump_read(const char* path, ecg_fcidump** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  return host_guarded([&] {
+    ECG_CHECK(path, "path must not be NULL");
+    auto f = std::make_unique<ecg_fcidump>();
+    f->fd = ecg::read_fcidump(path);
+    *out = f.release();
+    return 0;
+  });
+}
+
+int ecg_fcidump_free(ecg_fcidump* fd) {
+  if (!fd) return -1;
+  delete fd;
+  return 0;
+}
+
+int ecg_fcidump_info(const ecg_fcidump* f, int* norb, int* nelec, int* ms2, int* st, double* e0) {
+  if (!f) return -1;
+  return host_guarded([&] {
+    if (norb) *norb = f->fd.norb;
+    if (nelec) *nelec = (int)f->fd.head_int("NELEC", -1);
+    if (ms2) *ms2 = (int)f->fd.head_int("MS2", 0);
+    if (st) *st = (int)f->fd.head_int("ST", 0);
+    if (e0) *e0 = f->fd.int0;
+    return 0;
+  });
+}
+
+int ecg_fcidump_header(const ecg_fcidump* f, const char* key, char* buf, int buflen, char* type) {
+  if (!f || !key) return -1;
+  return host_guarded([&] {
+    auto it = f->fd.head.find(key);
+    if (it == f->fd.head.end()) return 1;
+    std::string s;
+    for (size_t k = 0; k < it->second.size(); ++k) s += (k ? "," : "") + it->second[k];
+    if (type) *type = f->fd.head_type.at(key);
+    return copy_text(s, buf, buflen);
+  });
+}
+
+int ecg_fcidump_header_key(const ecg_fcidump* f, int index, char* buf, int buflen) {
+  if (!f || index < 0) return -1;
+  return host_guarded([&] {
+    if (index >= (int)f->fd.head.size()) return 1;
+    auto it = f->fd.head.begin();
+    std::advance(it, index);
+    return copy_text(it->first, buf, buflen);
+  });
+}
+
+int ecg_fcidump_integrals(const ecg_fcidump* f, double* h, double* eri) {
+  if (!f) return -1;
+  if (h) std::memcpy(h, f->fd.h.data(), sizeof(double) * f->fd.h.size());
+  if (eri) std::memcpy(eri, f->fd.eri.data(), sizeof(double) * f->fd.eri.size());
+  return 0;
+}
+
+int ecg_fcidump_df(ecg_fcidump* f, double rtol, int* naux, double* BL, double* BR, int* has_br) {
+  if (!f) return -1;
+  return host_guarded([&] {
+    const ecg::DfFactors& d = fcidump_df(f, rtol);
+    if (naux) *naux = d.naux;
+    if (has_br) *has_br = d.BR.empty() ? 0 : 1;
+    if (BL) std::memcpy(BL, d.BL.data(), sizeof(double) * d.BL.size());
+    if (BR) std::memcpy(BR, (d.BR.empty() ? d.BL : d.BR).data(), sizeof(double) * d.BL.size());
+    return 0;
+  });
+}
+
+int ecg_dims(const ecg_ctx* ctx, int* nocc, int* nvirt, int* naux) {
+  if (!ctx) return -1;
+  if (nocc) *nocc = ctx->c.no;
+  if (nvirt) *nvirt = ctx->c.nv;
+  if (naux) *naux = ctx->c.nq;
+  return 0;
+}
+
+int ecg_create_from_fcidump(ecg_ctx** out, const char* path, double rtol, int device, size_t workspace_bytes) {
+  if (!out) return -1;
+  *out = nullptr;
+  ecg_fcidump* f = nullptr;
+  int st = ecg_fcidump_read(path, &f);
+  if (st != 0) return st;
+  std::unique_ptr<ecg_fcidump> hold(f);
+  st = host_guarded([&] {
+    const long nelec = f->fd.head_int("NELEC", -1);
+    ECG_CHECK(nelec >= 0, "NELEC option not found in fcidump");
+    ECG_CHECK(nelec % 2 == 0 && f->fd.head_int("MS2", 0) == 0,
+              "the closed-shell path needs an even NELEC and MS2 = 0");
+    const int nocc = (int)(nelec / 2);
+    ECG_CHECK(nocc > 0 && nocc < f->fd.norb, "NELEC / 2 must lie between 1 and NORB - 1");
+    fcidump_df(f, rtol);
+    return 0;
+  });
+  if (st != 0) return st;
+  const ecg::DfFactors& d = f->df;
+  const int n = f->fd.norb, nocc = (int)(f->fd.head_int("NELEC", 0) / 2);
+  return ecg_create(out, nocc, n - nocc, d.naux, f->fd.h.data(), d.BL.data(), d.BR.empty() ? nullptr : d.BR.data(),
+                    f->fd.int0, device, workspace_bytes);
 }
 
 int ecg_fock(ecg_ctx* ctx, double* fock, double* eps, double* ehf) {

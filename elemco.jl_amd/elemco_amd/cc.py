@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import lib
-from .fcidump import df_factors, read_fcidump
+from .fcidump import read_fcidump
 from .synthetic import make_system
 
 
@@ -68,7 +68,7 @@ class ECInfo:
     def from_fcidump(cls, path, **kw):
         """``ccdriver(EC, method; fcidump)`` input path (``dump.jl:326``)."""
         fd = read_fcidump(path)
-        BL, BR = df_factors(fd.eri, hermitian=not fd.simtra)
+        BL, BR = fd.df_factors()
         return cls(fd.nelec // 2, fd.int1, BL, BR, fd.int0, **kw)
 
     @classmethod

@@ -58,6 +58,16 @@ _SIGS = {
     "ecg_energy_grad": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_profile": [_vp, ctypes.c_int],
     "ecg_stats": [_vp, _dp],
+    "ecg_fcidump_read": [ctypes.c_char_p, ctypes.POINTER(_vp)],
+    "ecg_fcidump_free": [_vp],
+    "ecg_fcidump_info": [_vp, _ip, _ip, _ip, _ip, _dp],
+    "ecg_fcidump_header": [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p],
+    "ecg_fcidump_header_key": [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
+    "ecg_fcidump_integrals": [_vp, _dp, _dp],
+    "ecg_fcidump_df": [_vp, ctypes.c_double, _ip, _dp, _dp, _ip],
+    "ecg_create_from_fcidump": [ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_size_t],
+    "ecg_dims": [_vp, _ip, _ip, _ip],
 }
 _RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p}
 EXPORTED = tuple(_SIGS)
@@ -105,6 +115,64 @@ def fortran(a, shape=None):
     return np.asfortranarray(a)
 
 
+class Fcidump:
+    """A closed-shell FCIDUMP read by the library's host-side reader (``ecg_fcidump_read``:
+    ``read_fcidump``, ``src/integrals/dump.jl:326-346``; text or NPY2/NPY1 integrals)."""
+
+    def __init__(self, path):
+        lib = load()
+        handle = _vp()
+        if lib.ecg_fcidump_read(os.fsencode(path), ctypes.byref(handle)) != 0:
+            raise ElemcoError(f"ecg_fcidump_read failed: {lib.ecg_last_error(None).decode()}")
+        self._h, self._lib = handle, lib
+        norb, nelec, ms2, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        e0 = ctypes.c_double()
+        self._chk(lib.ecg_fcidump_info(handle, ctypes.byref(norb), ctypes.byref(nelec), ctypes.byref(ms2),
+                                       ctypes.byref(st), ctypes.byref(e0)))
+        self.norb, self.nelec, self.ms2, self.st, self.int0 = norb.value, nelec.value, ms2.value, st.value, e0.value
+        self.head = {}
+        buf = ctypes.create_string_buffer(1 << 16)
+        typ = ctypes.create_string_buffer(2)
+        k = 0
+        while lib.ecg_fcidump_header_key(handle, k, buf, len(buf)) == 0:
+            key = buf.value.decode()
+            self._chk(lib.ecg_fcidump_header(handle, key.encode(), buf, len(buf), typ))
+            conv = {b"i": int, b"f": float}.get(typ.value, str)
+            self.head[key] = [conv(x) for x in buf.value.decode().split(",")] if buf.value else []
+            k += 1
+        n = self.norb
+        self.int1 = np.zeros((n, n), order="F")
+        self.eri = np.zeros((n, n, n, n), order="F")
+        self._chk(lib.ecg_fcidump_integrals(handle, _ptr(self.int1), _ptr(self.eri)))
+
+    def _chk(self, st):
+        if st != 0:
+            raise ElemcoError(self._lib.ecg_last_error(None).decode())
+
+    def df_factors(self, rtol=0.0):
+        """Exact DF factors ``(BL, BR)`` (``BR`` None when it equals ``BL``) of the integrals
+        (``ecg_fcidump_df``)."""
+        naux, has_br = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.ecg_fcidump_df(self._h, float(rtol), ctypes.byref(naux), None, None, ctypes.byref(has_br)))
+        n = self.norb
+        BL = np.zeros((n, n, naux.value), order="F")
+        BR = np.zeros((n, n, naux.value), order="F") if has_br.value else None
+        self._chk(self._lib.ecg_fcidump_df(self._h, float(rtol), ctypes.byref(naux), _ptr(BL), _ptr(BR),
+                                           ctypes.byref(has_br)))
+        return BL, BR
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ecg_fcidump_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One library context: integrals resident on one GPU (``ecg_create``)."""
 
@@ -122,6 +190,23 @@ class Context:
             raise ElemcoError(f"ecg_create failed ({st}): {lib.ecg_last_error(None).decode()}")
         self._h = handle
         self._lib = lib
+
+    @classmethod
+    def from_fcidump(cls, path, rtol=0.0, device=0, workspace_bytes=0):
+        """Context straight from an FCIDUMP file (``ecg_create_from_fcidump``: reader, exact DF
+        factors and ``ecg_create`` with nocc = NELEC / 2 in one native call)."""
+        lib = load()
+        handle = _vp()
+        st = lib.ecg_create_from_fcidump(ctypes.byref(handle), os.fsencode(path), float(rtol), int(device),
+                                         int(workspace_bytes))
+        if st != 0:
+            raise ElemcoError(f"ecg_create_from_fcidump failed ({st}): {lib.ecg_last_error(None).decode()}")
+        self = cls.__new__(cls)
+        self._h, self._lib = handle, lib
+        o, v, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(lib.ecg_dims(handle, ctypes.byref(o), ctypes.byref(v), ctypes.byref(q)))
+        self.nocc, self.nvirt, self.naux, self.norb = o.value, v.value, q.value, o.value + v.value
+        return self
 
     def close(self):
         if getattr(self, "_h", None):

@@ -20,7 +20,7 @@ The drop-in methods at the end keep the reference's signatures:
 `ctx` stands for the reference's `EC::ECInfo` on this path: it owns the
 integrals resident on one GPU.  Build it from MO density-fitting factors
 `B[p,q,Q]` (the layout of `mmL`, `src/integrals/dftools.jl:102`) and the
-one-electron integrals.
+one-electron integrals, or straight from an FCIDUMP file (`ECGContext(path)`).
 """
 module ElemCoAMD
 
@@ -90,6 +90,26 @@ function ECGContext(nocc::Integer, h::Matrix{Float64}, BL::Array{Float64,3}; BR=
                              ref, nocc, n - nocc, naux, h, BL, brp, e0, device, 0)
   _check(nothing, st)
   ctx = ECGContext(ref[], nocc, n - nocc, naux)
+  finalizer(c -> (c.handle != C_NULL && ccall((:ecg_destroy, LIB), Cint, (Ptr{Cvoid},), c.handle); c.handle = C_NULL), ctx)
+  return ctx
+end
+
+"""
+    ECGContext(fcidump::AbstractString; rtol=0.0, device=0)
+
+`read_fcidump` (src/integrals/dump.jl:326-346; text or NPY2/NPY1 integrals, plain or
+gzip-compressed), exact DF factors and `ecg_create` with nocc = NELEC/2 in one native
+call (`ecg_create_from_fcidump`): the set-up `ccdriver(EC, method; fcidump)` does
+(src/cc/ccdriver.jl:37-76) for the closed-shell path.
+"""
+function ECGContext(fcidump::AbstractString; rtol=0.0, device=0)
+  ref = Ref{Ptr{Cvoid}}(C_NULL)
+  st = ccall((:ecg_create_from_fcidump, LIB), Cint, (Ref{Ptr{Cvoid}}, Cstring, Cdouble, Cint, Csize_t),
+             ref, fcidump, rtol, device, 0)
+  _check(nothing, st)
+  o, v, q = Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0)
+  ccall((:ecg_dims, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}, Ref{Cint}), ref[], o, v, q)
+  ctx = ECGContext(ref[], o[], v[], q[])
   finalizer(c -> (c.handle != C_NULL && ccall((:ecg_destroy, LIB), Cint, (Ptr{Cvoid},), c.handle); c.handle = C_NULL), ctx)
   return ctx
 end

@@ -52,7 +52,47 @@ void ecg_default_opts(ecg_solver_opts* opts);
 int ecg_create(ecg_ctx** ctx, int nocc, int nvirt, int naux, const double* h, const double* BL,
                const double* BR, double e0, int device, size_t workspace_bytes);
 int ecg_destroy(ecg_ctx* ctx);
+/* Message of the last failed call on ctx; with ctx = NULL, of the last failed
+ * ecg_create* or ecg_fcidump_* call of the calling thread. */
 const char* ecg_last_error(const ecg_ctx* ctx);
+
+/* FCIDUMP input, host side (no GPU): read_fcidump (src/integrals/dump.jl:326-346)
+ * for closed-shell (IUHF = 0) files, plain or gzip-compressed.  The header
+ * (read_header, dump.jl:360-405) is kept with the reference's typing; integrals
+ * come from the NPY files named by NPY2/NPY1 when present (Fortran-ordered
+ * float64, int2 = <pq|rs> at [p,q,r+s(s+1)/2], r <= s; core energy ENUC;
+ * dump.jl:442-461, 731-745; a missing file falls back to the integral lines) or
+ * from the integral lines (read_integrals!, dump.jl:573-604; set_int1!/set_int2!,
+ * dump.jl:472-544: particle symmetry always, 8-fold symmetry unless ST > 0).
+ * Errors: message in ecg_last_error(NULL). */
+typedef struct ecg_fcidump ecg_fcidump;
+int ecg_fcidump_read(const char* path, ecg_fcidump** fd);
+int ecg_fcidump_free(ecg_fcidump* fd);
+/* NORB, NELEC (-1 if absent), MS2 (0 if absent), ST and the core energy; any
+ * output may be NULL. */
+int ecg_fcidump_info(const ecg_fcidump* fd, int* norb, int* nelec, int* ms2, int* st, double* e0);
+/* headvar (dump.jl:664-696): the values of header key `key` (upper case) joined by
+ * ',' into buf (NUL-terminated), and its type ('i' Int, 'f' Float64, 's' String) in
+ * *type.  Returns 1 if the key is absent, -2 if buflen is too small. */
+int ecg_fcidump_header(const ecg_fcidump* fd, const char* key, char* buf, int buflen, char* type);
+/* Name of header key number `index` (0-based, sorted); 1 past the last key. */
+int ecg_fcidump_header_key(const ecg_fcidump* fd, int index, char* buf, int buflen);
+/* h[p,q] (n x n) and (pq|rs) (n^4, chemists' notation, column-major); either may be NULL. */
+int ecg_fcidump_integrals(const ecg_fcidump* fd, double* h, double* eri);
+/* Exact DF factors of the integrals: (pq|rs) = sum_Q BL[p,q,Q] BR[r,s,Q] by
+ * diagonally pivoted LDL^T of the (pq),(rs) supermatrix (Cholesky for ST = 0),
+ * stopped at pivots <= rtol * the largest diagonal (rtol <= 0: 1e-14).  Computed on
+ * the first call and kept with fd; a call with BL = NULL only returns *naux and
+ * *has_br (BR differs from BL: ST > 0 with negative pivots), so BL and BR are
+ * n x n x naux arrays the caller allocates.  No reference counterpart: the
+ * reference keeps four-index integrals; the GPU path takes factors. */
+int ecg_fcidump_df(ecg_fcidump* fd, double rtol, int* naux, double* BL, double* BR, int* has_br);
+/* ecg_fcidump_read + ecg_fcidump_df + ecg_create with nocc = NELEC / 2: the
+ * ccdriver(EC, method; fcidump) set-up (src/cc/ccdriver.jl:37-76) in one call. */
+int ecg_create_from_fcidump(ecg_ctx** ctx, const char* path, double rtol, int device, size_t workspace_bytes);
+/* Orbital-space sizes of a context (the reference's EC.space 'o', 'v' and the DF
+ * basis, src/infos/ecinfos.jl:231-244); any output may be NULL. */
+int ecg_dims(const ecg_ctx* ctx, int* nocc, int* nvirt, int* naux);
 
 /* Closed-shell Fock matrix (n x n), orbital energies (n) and HF energy:
  * gen_fock (src/scf/fockfactory.jl:27-30), calc_fock_matrix / calc_HF_energy

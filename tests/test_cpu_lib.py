@@ -62,15 +62,22 @@ def test_create_without_gpu_fails_loudly():
 
 
 def test_df_factors_reproduce_fcidump_integrals():
-    """Exact factorisation of the reference's H2O fixture (Hermitian) and ST1 (signed factors)."""
-    from elemco_amd.fcidump import df_factors, read_fcidump
+    """Exact factorisation of the reference's H2O fixture (Hermitian: one factor) and ST1
+    (non-Hermitian: signed factors, BR != BL)."""
+    from elemco_amd.fcidump import read_fcidump
     for name, herm in (("H2O.FCIDUMP.gz", True), ("H2O_ST1.FCIDUMP.gz", False)):
         fd = read_fcidump(os.path.join(ROOT, "tests", "golden", name))
-        BL, BR = df_factors(fd.eri, hermitian=not fd.simtra)
+        assert fd.simtra == (not herm)
+        BL, BR = fd.df_factors()
         assert (BR is None) == herm
         BRx = BL if BR is None else BR
         rec = np.einsum("pqQ,rsQ->pqrs", BL, BRx)
         assert np.max(np.abs(rec - fd.eri)) < 1e-12
+        # a coarser tolerance keeps fewer factors and reconstructs to about that tolerance
+        BL2, BR2 = fd.df_factors(1e-8)
+        assert BL2.shape[2] < BL.shape[2]
+        rec2 = np.einsum("pqQ,rsQ->pqrs", BL2, BL2 if BR2 is None else BR2)
+        assert np.max(np.abs(rec2 - fd.eri)) < 1e-6
 
 
 def test_product_fcidump_reader_matches_oracle():

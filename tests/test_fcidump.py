@@ -1,0 +1,117 @@
+"""Host-side FCIDUMP reader of the library (ecg_fcidump_*; reference: src/integrals/dump.jl).
+
+The reference's own fixtures (test/files/H2O.FCIDUMP, H2O_ST1.FCIDUMP, stored
+gzip-compressed under tests/golden) pin the text path against the oracle reader;
+the NPY path (NPY2/NPY1 + ENUC, dump.jl:442-461) is pinned against the text path
+on the same integrals written in the reference's NPY layout (<pq|rs> at
+[p, q, r + s(s+1)/2], r <= s, Fortran order).  No GPU needed.
+"""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from elemco_amd import lib
+from elemco_amd.fcidump import read_fcidump
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _npy_layout(eri):
+    """(pq|rs) -> the reference's NPY int2: <pq|rs> = (pr|qs) at [p, q, r + s(s+1)/2], r <= s."""
+    n = eri.shape[0]
+    int2 = np.zeros((n, n, n * (n + 1) // 2), order="F")
+    for s in range(n):
+        for r in range(s + 1):
+            int2[:, :, r + s * (s + 1) // 2] = eri[:, r, :, s]
+    return int2
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+@pytest.mark.parametrize("name", ["H2O.FCIDUMP.gz", "H2O_ST1.FCIDUMP.gz"])
+def test_npy_integrals_match_text(tmp_path, name):
+    fd = read_fcidump(os.path.join(GOLDEN, name))
+    n = fd.norb
+    np.save(tmp_path / "int2.npy", np.asfortranarray(_npy_layout(fd.eri)))
+    np.save(tmp_path / "int1.npy", np.asfortranarray(fd.int1))
+    st = fd.head["ST"][0]
+    _write(tmp_path / "FCIDUMP", f" &FCI NORB={n},NELEC={fd.nelec},MS2=0,\n  ST={st},\n"
+           f"  NPY2='int2.npy', NPY1=\"int1.npy\",\n  ENUC={fd.int0!r},\n /\n")
+    got = read_fcidump(str(tmp_path / "FCIDUMP"))
+    assert got.head["NPY2"] == ["int2.npy"] and got.head["NPY1"] == ["int1.npy"]
+    assert np.array_equal(got.eri, fd.eri)
+    assert np.array_equal(got.int1, fd.int1)
+    assert got.int0 == fd.int0
+
+
+def test_npy_missing_file_falls_back_to_text(tmp_path):
+    """mmap_integrals (dump.jl:731-745): a missing NPY file means the integral lines are read."""
+    with gzip.open(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"), "rt") as f:
+        text = f.read()
+    head, rest = text.split("/", 1) if text.lstrip().startswith("&") else (None, None)
+    assert head is not None
+    _write(tmp_path / "FCIDUMP", head.rstrip() + "\n NPY2='absent.npy', NPY1='absent1.npy', ENUC=0.0,\n/" + rest)
+    ref = read_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"))
+    got = read_fcidump(str(tmp_path / "FCIDUMP"))
+    assert np.array_equal(got.eri, ref.eri) and got.int0 == ref.int0
+
+
+def test_npy_c_order_is_refused(tmp_path):
+    n = 2
+    np.save(tmp_path / "int2.npy", np.ascontiguousarray(np.ones((n, n, 3))))
+    np.save(tmp_path / "int1.npy", np.asfortranarray(np.eye(n)))
+    _write(tmp_path / "FCIDUMP", f"&FCI NORB={n},NELEC=2,\nNPY2='int2.npy',NPY1='int1.npy',ENUC=1.0,\n/\n")
+    with pytest.raises(lib.ElemcoError, match="C-ordered"):
+        read_fcidump(str(tmp_path / "FCIDUMP"))
+
+
+def test_header_typing_and_case(tmp_path):
+    """read_header (dump.jl:360-405): values typed by the first one, keys upper-cased."""
+    _write(tmp_path / "FCIDUMP", "&FCI norb=2,NELEC=2,MS2=0,\n ORBSYM=1,1,\n isym=1,\n NAME='h2 test', X=0.5,1,\n"
+           "&END\n 0.5 1 1 1 1\n 0.25 2 2 1 1\n -1.0 1 1 0 0\n -0.5 2 1 0 0\n 0.7 0 0 0 0\n")
+    fd = read_fcidump(str(tmp_path / "FCIDUMP"))
+    assert fd.head["NORB"] == [2] and fd.head["ORBSYM"] == [1, 1] and fd.head["ISYM"] == [1]
+    assert fd.head["NAME"] == ["h2", "test"]          # blanks split values, quotes are stripped
+    assert fd.head["X"] == [0.5, 1.0]
+    assert fd.head["ST"] == [0] and fd.head["IUHF"] == [0]   # defaults
+    assert fd.int0 == 0.7
+    # 8-fold symmetry (ST = 0) of the two-electron lines, symmetric h
+    assert fd.eri[1, 1, 0, 0] == fd.eri[0, 0, 1, 1] == 0.25
+    assert fd.int1[1, 0] == fd.int1[0, 1] == -0.5 and fd.int1[0, 0] == -1.0
+
+
+def test_simtra_keeps_particle_symmetry_only(tmp_path):
+    _write(tmp_path / "FCIDUMP", "&FCI NORB=2,NELEC=2,ST=1,\n/\n 0.3 1 2 1 1\n 0.4 2 1 0 0\n 0.1 0 0 0 0\n")
+    fd = read_fcidump(str(tmp_path / "FCIDUMP"))
+    assert fd.simtra
+    assert fd.eri[0, 1, 0, 0] == fd.eri[0, 0, 0, 1] == 0.3     # (12|11) = (11|12)
+    assert fd.eri[1, 0, 0, 0] == 0.0                            # (21|11) not implied
+    assert fd.int1[1, 0] == 0.4 and fd.int1[0, 1] == 0.0
+
+
+@pytest.mark.parametrize("body,msg", [
+    ("&FCI NORB=2,NELEC=2,\n/\n 0.5 3 1 1 1\n 0.1 0 0 0 0\n", "Index larger than norb"),
+    ("&FCI NORB=2,NELEC=2,\n/\n 0.5 1 1 1 1\n", "No core energy"),
+    ("&FCI NORB=2,NELEC=2,IUHF=1,\n/\n 0.1 0 0 0 0\n", "IUHF"),
+    ("&FCI NORB=2,NELEC=2,\n 0.1 0 0 0 0\n", "not terminated"),
+    ("&FCI NELEC=2,\n/\n 0.1 0 0 0 0\n", "NORB"),
+])
+def test_reader_errors(tmp_path, body, msg):
+    _write(tmp_path / "FCIDUMP", body)
+    with pytest.raises(lib.ElemcoError, match=msg):
+        read_fcidump(str(tmp_path / "FCIDUMP"))
+
+
+def test_df_of_indefinite_integrals_needing_two_by_two_pivots_is_an_error(tmp_path):
+    """A supermatrix with zero diagonal and nonzero off-diagonal entries, (11|22) = (22|11) = 1
+    and everything else 0 (eigenvalues +-1), has no 1x1 pivot: an error, not a wrong factor."""
+    _write(tmp_path / "FCIDUMP", "&FCI NORB=2,NELEC=2,ST=1,\n/\n 1.0 1 1 2 2\n 0.0 0 0 0 0\n")
+    fd = read_fcidump(str(tmp_path / "FCIDUMP"))
+    with pytest.raises(lib.ElemcoError, match="2x2 pivots"):
+        fd.df_factors()

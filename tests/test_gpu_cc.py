@@ -112,6 +112,25 @@ def test_h2o_fcidump_ccsd():
     assert abs(e["CCSDc"] - eo["CCSDc"]) < 1e-9
 
 
+@pytest.mark.parametrize("name,method,eref", [("H2O.FCIDUMP.gz", E.ECG_CCSD, None),
+                                              ("H2O_ST1.FCIDUMP.gz", E.ECG_DCSD, -0.082498102641)])
+def test_create_from_fcidump_one_call(name, method, eref):
+    """ecg_create_from_fcidump (native reader + exact DF factors + ecg_create) gives the
+    energies of the host-mirror path (ECInfo.from_fcidump) and of the reference (h2o_st1.jl:9)."""
+    from elemco_amd.lib import Context
+    path = os.path.join(GOLDEN, name)
+    ctx = Context.from_fcidump(path)
+    EC = E.ECInfo.from_fcidump(path)
+    assert (ctx.nocc, ctx.nvirt, ctx.naux) == (EC.ctx.nocc, EC.ctx.nvirt, EC.ctx.naux)
+    opts = EC.opts()
+    e1 = ctx.cc_solve(method, E.ECG_FP64, opts)[0]
+    e2 = EC.ctx.cc_solve(method, E.ECG_FP64, opts)[0]
+    assert abs(float(e1[0]) - float(e2[0])) < 1e-12
+    if eref is not None:
+        assert abs(float(e1[0]) - eref) < 1e-6
+    ctx.close()
+
+
 @pytest.mark.parametrize("method,eref", [("ccsd", -0.082041632192), ("dcsd", -0.082498102641)])
 def test_h2o_st1_fcidump(method, eref):
     """Non-Hermitian fixture (test/h2o_st1.jl:9): two-sided DF factors."""
