@@ -84,6 +84,9 @@ struct PackedOp {
   // write C at + z * c_z (elements)
   int batch = 1;
   long c_z = 0;
+  // f32: the planes are FP32 (pack_rows_f32; Ah / Bh hold the FP32 plane pointers, Al / Bl
+  // are unused, row strides and batch offsets count floats) and the products are FP32 MFMA
+  bool f32 = false;
 };
 
 // Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,
@@ -116,7 +119,10 @@ struct GemmEngine {
   // Rows are written up to the next multiple of 64.
   void pack_rows_bf16x3(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad,
                         uint16_t* hi, uint16_t* lo);
-  void run_bf16x3_packed(const PackedOp& op);
+  void run_bf16x3_packed(const PackedOp& op);   // also the FP32 packed product (op.f32)
+  // Pack rows as pack_rows_bf16x3 into one FP32 plane [r][kpad]; within each 32-wide K slab
+  // position 8*(k%4) + k/4 holds k (the order the FP32 packed product reads).
+  void pack_rows_f32(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad, float* out);
   void reduce_slabs(const GemmOp& op, int ksplit);   // split-K partial slabs in ws -> C
   hipEvent_t timing_begin();
   void timing_end(hipEvent_t e0);

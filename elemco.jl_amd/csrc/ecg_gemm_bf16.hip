@@ -17,6 +17,12 @@
 // MFMA operand layout (16x16x32 BF16, gfx950): A[i][k] in lane i + 16*(k/8),
 // element k%8; B[k][j] in lane j + 16*(k/8), element k%8; D[i][j] in lane
 // j + 16*(i/4), slot i%4.
+//
+// The same machinery serves FP32 products on packed planes (one FP32 plane per
+// operand, v_mfma_f32_16x16x4f32: A[i][k] in lane i + 16*k, k < 4; D as above):
+// pack_f32p_kernel stores each 32-wide K slab permuted, position 8*(k%4) + k/4
+// holding k, so the eight k values a lane needs for the slab's eight MFMA steps
+// (k = 4s + lane/16, s < 8) are two consecutive 16-byte chunks.
 #include "ecg_gemm.h"
 #include "ecg_gemm_dev.h"
 
@@ -43,6 +49,7 @@ struct PackArgs {
   int d0;
   u16* hi;   // [rows_pad][kpad]
   u16* lo;
+  float* f32;   // FP32 plane [rows_pad][kpad] (pack_f32p_kernel)
 };
 
 __device__ __forceinline__ u16 bf16_bits(float f) {
@@ -93,6 +100,42 @@ __global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p, int nrt) {
   const size_t off = (size_t)(r0 + wr) * p.kpad + k0 + 8 * ch;
   *reinterpret_cast<u32x4_t*>(p.hi + off) = vh;
   *reinterpret_cast<u32x4_t*>(p.lo + off) = vl;
+}
+
+// FP32 plane with the slab permutation (position 8*(k%4) + k/4 of each 32-wide slab holds
+// k); grid and reads as pack_bf16x3_kernel.
+__global__ __launch_bounds__(256) void pack_f32p_kernel(PackArgs p, int nrt) {
+  __shared__ float fs[XK][PR + 1];
+  const int t = threadIdx.x;
+  const long bid = blockIdx.x;
+  const int r0 = (int)(bid % nrt) * PR, k0 = (int)(bid / nrt) * XK;
+  const double* src = p.src[0];
+  const long sk = p.sk[0];
+  const int klen = p.sK[0];
+  const int row = t & (PR - 1);
+  const int r = r0 + row;
+  const bool rok = r < p.rows;
+  const long roff = rok ? split_off(r, p.d0, p.s0, p.s1) : 0;
+#pragma unroll
+  for (int j = 0; j < XK / 4; ++j) {
+    const int kk = (t >> 6) + 4 * j;
+    const int k = k0 + kk;
+    const bool ok = rok && k < klen;
+    const double x = src[ok ? roff + (long)k * sk : 0];
+    fs[kk][row] = ok ? (float)x : 0.f;
+  }
+  __syncthreads();
+  // write: row r0 + t/4, positions 8q .. 8q+7 (q = t%4) = k = q, q+4, ..., q+28
+  const int wr = t >> 2, q = t & 3;
+  f4_t v0, v1;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v0[e] = fs[q + 4 * e][wr];
+    v1[e] = fs[q + 4 * (e + 4)][wr];
+  }
+  float* dst = p.f32 + (size_t)(r0 + wr) * p.kpad + k0 + 8 * q;
+  *reinterpret_cast<f4_t*>(dst) = v0;
+  *reinterpret_cast<f4_t*>(dst + 4) = v1;
 }
 
 struct X3Params {
@@ -470,6 +513,145 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
   x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
 }
 
+// 16 bytes per lane from global memory at base + off (base uniform, in SGPRs; off a
+// per-lane byte offset) into LDS at M0 + 16 * lane; otherwise as dma16.
+__device__ __forceinline__ void dma16s(const void* base, unsigned off, lds_vptr dst) {
+  const unsigned m0 = (unsigned)(__UINTPTR_TYPE__)dst;
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(m0), "v"(off), "s"(base)
+               : "memory");
+}
+
+// FP32 planes and row strides (in floats) of the segment holding slab k0: the packed
+// FP32 product keeps its planes in X3Params::Ah / Bh (as FP32 pointers).
+struct SlabSrcF {
+  const float *a, *b;
+  long lda, ldb;
+  int kl;
+};
+template <bool MS>
+__device__ __forceinline__ SlabSrcF slab_src_f32(const X3Params& p, int k0, long zb) {
+  SlabSrcF r{reinterpret_cast<const float*>(p.Ah[0]), reinterpret_cast<const float*>(p.Bh[0]), p.kpad, p.kpad, k0};
+  if constexpr (MS) {
+    r.lda = p.lda[0]; r.ldb = p.ldb[0];
+    long oa = p.az[0], ob = p.bz[0];
+#pragma unroll
+    for (int q = 1; q < MAXSEG; ++q)
+      if (q < p.nseg && k0 >= p.kb[q]) {
+        r.a = reinterpret_cast<const float*>(p.Ah[q]); r.b = reinterpret_cast<const float*>(p.Bh[q]);
+        r.lda = p.lda[q]; r.ldb = p.ldb[q]; r.kl = k0 - p.kb[q];
+        oa = p.az[q]; ob = p.bz[q];
+      }
+    r.a += zb * oa; r.b += zb * ob;
+  }
+  return r;
+}
+
+// FP32 product on packed FP32 planes (pack_f32p_kernel): the tile, waves, LDS buffers,
+// direct HBM -> LDS slab loads and epilogue of gemm_bf16x3_dma_kernel; a slab row is 32
+// floats = 8 chunks of 16 bytes, chunk c of row r stored at slot c ^ (r % 8) (the rows that
+// a 16-lane read touches then hit distinct bank groups).  Per slab and wave: 16 reads of
+// 16 bytes and 128 v_mfma_f32_16x16x4f32 (eight k steps of the 4x4 tiles).
+template <bool TO, bool MS>
+__global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
+  __shared__ u32x4_t A0[XM][8], A1[XM][8], A2[XM][8];   // [row][16-byte chunk]
+  __shared__ u32x4_t B0[XN][8], B1[XN][8], B2[XN][8];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave index in an SGPR
+  const int wm = wave & 3, wn = wave >> 2;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
+  const int bm = tm * XM, bn = tn * XN;
+  const int kz = blockIdx.z;
+  const long zb = blockIdx.y;
+  const int kbeg = kz * p.kchunk;
+  const int kend = min(p.kpad, kbeg + p.kchunk);
+
+  // slab k0 -> one buffer: A is 32 blocks of 8 rows x 128 bytes, B 16 blocks; wave w moves
+  // blocks w, w+8, ... (4 of A, 2 of B).  Lane l of a block loads row 8 c + l/8 into slot
+  // l%8, i.e. chunk (l%8) ^ (l/8) of that row: a uniform block base plus a per-lane byte
+  // offset that is the same for every block of an operand.
+  const int lrow = lane >> 3, lch = (lane & 7) ^ lrow;
+  auto issue = [&](u32x4_t (&As)[XM][8], u32x4_t (&Bs)[XN][8], int k0) {
+    const SlabSrcF g = slab_src_f32<MS>(p, k0, zb);
+    const unsigned offa = (unsigned)(lrow * g.lda + 4 * lch) * 4u;
+    const unsigned offb = (unsigned)(lrow * g.ldb + 4 * lch) * 4u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r0 = 8 * (wave + 8 * j);
+      dma16s(g.a + (size_t)(bm + r0) * g.lda + g.kl, offa, (lds_vptr)&As[r0][0]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int r0 = 8 * (wave + 8 * j);
+      dma16s(g.b + (size_t)(bn + r0) * g.ldb + g.kl, offb, (lds_vptr)&Bs[r0][0]);
+    }
+  };
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+  const int li = lane & 15, lk = lane >> 4;
+  // a slab in two halves of four k steps (chunks 2 lk + h): 32 fragment registers live
+  auto compute = [&](const u32x4_t (&As)[XM][8], const u32x4_t (&Bs)[XN][8]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      f4_t a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wm * 64 + i * 16 + li;
+        a[i] = __builtin_bit_cast(f4_t, As[r][(2 * lk + h) ^ (r & 7)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = wn * 64 + j * 16 + li;
+        b[j] = __builtin_bit_cast(f4_t, Bs[c][(2 * lk + h) ^ (c & 7)]);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            if constexpr (TO) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(b[j][s], a[i][s], acc[i][j], 0, 0, 0);
+            else acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+          }
+    }
+  };
+  auto slab = [&](int k0) { return k0 < kend ? k0 : kbeg; };
+  auto wait_prev = []() { asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); };
+  auto barrier = []() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+  if (kbeg < kend) {
+    issue(A0, B0, kbeg);
+    issue(A1, B1, slab(kbeg + XK));
+    wait_prev();
+  }
+  barrier();
+  for (int k0 = kbeg; k0 < kend; k0 += 3 * XK) {
+    issue(A2, B2, slab(k0 + 2 * XK));
+    compute(A0, B0);
+    wait_prev();
+    barrier();
+    if (k0 + XK >= kend) break;
+    issue(A0, B0, slab(k0 + 3 * XK));
+    compute(A1, B1);
+    wait_prev();
+    barrier();
+    if (k0 + 2 * XK >= kend) break;
+    issue(A1, B1, slab(k0 + 4 * XK));
+    compute(A2, B2);
+    wait_prev();
+    barrier();
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
+  x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
+}
+
 }  // namespace
 
 void GemmEngine::run_bf16x3(const GemmOp& op) {
@@ -569,6 +751,19 @@ void GemmEngine::pack_rows_bf16x3(const double* src, long rows, int d0, long s0,
   ECG_HIP(hipGetLastError());
 }
 
+void GemmEngine::pack_rows_f32(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad,
+                               float* out) {
+  ECG_CHECK(K > 0 && kpad % XK == 0 && kpad >= K, "pack: kpad must be a multiple of 32 covering K");
+  const long rt = (rows + PR - 1) / PR;
+  ECG_CHECK(rt * PR < (1L << 31) && rt * (kpad / XK) < (1L << 31), "pack: operand too large for the packing grid");
+  PackArgs pa{};
+  pa.rows = (int)rows; pa.kpad = kpad; pa.nseg = 1;
+  for (int s = 0; s < MAXSEG; ++s) { pa.src[s] = src; pa.sk[s] = sk; pa.sK[s] = K; }
+  pa.s0 = s0; pa.s1 = s1; pa.d0 = d0; pa.f32 = out;
+  hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)(rt * (kpad / XK))), dim3(256), 0, stream, pa, (int)rt);
+  ECG_HIP(hipGetLastError());
+}
+
 void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   ECG_CHECK(op.nseg >= 1 && op.nseg <= MAXSEG, "gemm: packed product needs 1..4 K segments");
   ECG_CHECK((op.C != nullptr) != (op.Cf != nullptr), "gemm: packed product needs exactly one output");
@@ -579,7 +774,8 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   int kpad = 0;
   for (int s = 0; s < op.nseg; ++s) {
     const PackedSeg& g = op.seg[s];
-    ECG_CHECK(g.K > 0 && g.K % XK == 0 && g.Ah && g.Al && g.Bh && g.Bl && g.lda >= g.K && g.ldb >= g.K,
+    ECG_CHECK(g.K > 0 && g.K % XK == 0 && g.Ah && g.Bh && (op.f32 || (g.Al && g.Bl)) && g.lda >= g.K &&
+                  g.ldb >= g.K,
               "gemm: packed segment needs K a positive multiple of 32 and row strides >= K");
     x.kb[s] = kpad;
     x.Ah[s] = g.Ah; x.Al[s] = g.Al; x.lda[s] = g.lda;
@@ -613,7 +809,10 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
   const bool to = op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1);
   const dim3 grid((unsigned)tiles, (unsigned)op.batch, 1);
-  if (bf16_kernel == 1) {
+  if (op.f32) {
+    if (to) hipLaunchKernelGGL((gemm_f32p_dma_kernel<true, true>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_f32p_dma_kernel<false, true>), grid, dim3(512), 0, stream, x);
+  } else if (bf16_kernel == 1) {
     if (to) hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<true, true>), grid, dim3(512), 0, stream, x);
     else hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, true>), grid, dim3(512), 0, stream, x);
   } else {

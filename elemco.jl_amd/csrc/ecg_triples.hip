@@ -339,26 +339,34 @@ void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const do
   }
 }
 
-// BF16 hi/lo planes of the (T) operands, packed once per evaluation so that each
-// triplet's three products read them directly (no per-triplet packing).  Every
-// family is a stack of blocks with K-contiguous rows:
+// Packed planes of the (T) operands (BF16 hi/lo, or one FP32 plane for FP32 products),
+// packed once per evaluation so that each triplet's three products read them directly
+// (no per-triplet packing).  Every family is a stack of blocks with K-contiguous rows:
 //   W [m][(x,y)][d] = W[x,y,d,m]     Wt[m][(x,y)][d] = W[y,x,d,m]   (K = v)
 //   T3[x][(p,q)][l] = T[p,q,x,l]     T4[x][(p,q)][l] = T[p,q,l,x]   (K = o)
 //   Ts[(i,j)][a][d] = T[a,d,i,j]                                     (K = v)
 //   Hn[(m,n)][c][l] = -H[l,c,m,n]                                    (K = o)
 struct Planes {
   DevBuf* buf = nullptr;     // storage owned by the context (grown on demand)
+  bool f32 = false;          // one FP32 plane (f) instead of BF16 hi / lo
   uint16_t* hi = nullptr;
   uint16_t* lo = nullptr;
+  float* f = nullptr;
   long ld = 0;
   // rows of src as in GemmEngine::pack_rows_bf16x3; 256 readable rows are kept after the last
   void make(GemmEngine& g, const double* src, long rows, int d0, long s0, long s1, int K, long sk) {
     ld = ((K + 31) / 32) * 32;
     const long rows_alloc = ((rows + 63) / 64) * 64 + 256;
-    buf->ensure(sizeof(uint16_t) * 2 * rows_alloc * ld);
+    const long packed = ((rows + 63) / 64) * 64;
+    buf->ensure(sizeof(uint16_t) * 2 * rows_alloc * ld);   // 4 bytes per element either way
+    if (f32) {
+      f = static_cast<float*>(buf->p);
+      ECG_HIP(hipMemsetAsync(f + packed * ld, 0, sizeof(float) * (rows_alloc - packed) * ld, g.stream));
+      g.pack_rows_f32(src, rows, d0, s0, s1, K, sk, (int)ld, f);
+      return;
+    }
     hi = static_cast<uint16_t*>(buf->p);
     lo = hi + rows_alloc * ld;
-    const long packed = ((rows + 63) / 64) * 64;
     ECG_HIP(hipMemsetAsync(hi + packed * ld, 0, sizeof(uint16_t) * (rows_alloc - packed) * ld, g.stream));
     ECG_HIP(hipMemsetAsync(lo + packed * ld, 0, sizeof(uint16_t) * (rows_alloc - packed) * ld, g.stream));
     g.pack_rows_bf16x3(src, rows, d0, s0, s1, K, sk, (int)ld, hi, lo);
@@ -367,21 +375,24 @@ struct Planes {
 struct PackedInts { Planes W, Wt, Hn; };
 struct PackedAmps { Planes T3, T4, Ts; };
 
-void pack_ints(GemmEngine& g, int v, int o, const TripleInts& I, PackedInts& P) {
+void pack_ints(GemmEngine& g, int v, int o, const TripleInts& I, bool f32, PackedInts& P) {
   const long vv = (long)v * v, vvv = vv * v;
+  P.W.f32 = P.Wt.f32 = P.Hn.f32 = f32;
   P.W.make(g, I.W, (long)o * vv, (int)vv, 1, vvv, v, vv);
   P.Wt.make(g, I.Wt, (long)o * vv, (int)vv, 1, vvv, v, vv);
   P.Hn.make(g, I.Hn, (long)o * o * v, v, 1, (long)v * o, o, v);
 }
 
-void pack_amps(GemmEngine& g, int v, int o, const double* T, PackedAmps& P) {
+void pack_amps(GemmEngine& g, int v, int o, const double* T, bool f32, PackedAmps& P) {
   const long vv = (long)v * v;
+  P.T3.f32 = P.T4.f32 = P.Ts.f32 = f32;
   P.T3.make(g, T, (long)o * vv, 1 << 30, 1, 0, o, vv * o);
   P.T4.make(g, T, (long)o * vv, (int)vv, 1, vv * o, o, vv);
   P.Ts.make(g, T, (long)o * o * v, v, 1, vv, v, v);
 }
 
-// K[a,b,c] (FP32) for the triplets (i0 + z, j, k), z < nb, from packed planes: the
+// K[a,b,c] (FP32) for the triplets (i0 + z, j, k), z < nb, from packed planes (BF16 hi/lo
+// or FP32 products): the
 // same twelve products as build_K, each class with the pair of K indices carried
 // by the integral (or amplitude pair) operand as M (v^2 rows) and the single index
 // as N (v), so all three launches have the shape v^2 x v x (2v + 2o), batched over
@@ -396,11 +407,19 @@ void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, c
   // operand offsets as functions of i; the batch advances them by their change per unit i
   auto seg = [&](PackedSeg& s, const Planes& A, auto aoff, const Planes& B, auto boff) {
     const long a0 = aoff(i0), b0 = boff(i0);
-    s.Ah = A.hi + a0; s.Al = A.lo + a0; s.lda = A.ld; s.a_z = aoff(i0 + 1) - a0;
-    s.Bh = B.hi + b0; s.Bl = B.lo + b0; s.ldb = B.ld; s.b_z = boff(i0 + 1) - b0;
+    if (A.f32) {   // FP32 planes: the plane pointers travel in Ah / Bh (PackedOp::f32)
+      s.Ah = reinterpret_cast<const uint16_t*>(A.f + a0); s.Al = nullptr;
+      s.Bh = reinterpret_cast<const uint16_t*>(B.f + b0); s.Bl = nullptr;
+    } else {
+      s.Ah = A.hi + a0; s.Al = A.lo + a0;
+      s.Bh = B.hi + b0; s.Bl = B.lo + b0;
+    }
+    s.lda = A.ld; s.a_z = aoff(i0 + 1) - a0;
+    s.ldb = B.ld; s.b_z = boff(i0 + 1) - b0;
     s.K = (int)A.ld;
   };
   PackedOp op;
+  op.f32 = T.Ts.f32;
   op.M = (int)vv; op.N = v; op.nseg = 4; op.Cf = K[0];
   op.algorithmic_k = 2LL * v + 2LL * o;
   op.batch = nb; op.c_z = kz;
@@ -512,8 +531,9 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
       IL = TripleInts{vvvoL.d(), vvvoLT.d(), ovooLN.d()};
     }
   }
-  // BF16 hi/lo products run on operands packed once for all triplets, with K in FP32
-  const bool packed = (pr == Prec::BF16X3);
+  // BF16 hi/lo and FP32 products run on operands packed once for all triplets, with K in FP32
+  const bool packed = (pr == Prec::BF16X3 || pr == Prec::F32);
+  const bool f32 = (pr == Prec::F32);
   PackedInts PI, PIL;
   PackedAmps PT, PU;
   PI.W.buf = &W[11]; PI.Wt.buf = &W[12]; PI.Hn.buf = &W[13];
@@ -521,11 +541,11 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   PU.T3.buf = &W[17]; PU.T4.buf = &W[18]; PU.Ts.buf = &W[19];
   PIL.W.buf = &W[20]; PIL.Wt.buf = &W[21]; PIL.Hn.buf = &W[22];
   if (packed) {
-    pack_ints(ops.gemm, v, o, IT, PI);
-    pack_amps(ops.gemm, v, o, dT2.d(), PT);
+    pack_ints(ops.gemm, v, o, IT, f32, PI);
+    pack_amps(ops.gemm, v, o, dT2.d(), f32, PT);
     if (lambda) {
-      pack_amps(ops.gemm, v, o, dU2.d(), PU);
-      if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, PIL);
+      pack_amps(ops.gemm, v, o, dU2.d(), f32, PU);
+      if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, f32, PIL);
     }
   }
   const PackedInts& PIU = (lambda && !c.hermitian) ? PIL : PI;
@@ -641,7 +661,7 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   const long nrange = count < 0 ? ntrip_all : std::min<long>(count, std::max<long>(0, ntrip_all - first));
   int nbatch = 1;
   if (packed) {
-    int& tuned = c.triples_batch[lambda ? 1 : 0];
+    int& tuned = c.triples_batch[f32 ? 1 : 0][lambda ? 1 : 0];
     const long tiles = (((long)v * v + 255) / 256) * ((v + 127) / 128);
     nbatch = (int)std::max<long>(1, std::min<long>(BMAX, 1024 / std::max<long>(1, tiles)));
     if (tuned > 0) {

@@ -64,7 +64,8 @@ def test_h2o_ccsd_t_reference_energy():
 @pytest.mark.parametrize("cfg", [(5, 19, 72, 0), (4, 130, 100, 6), (6, 41, 90, 8)])
 @pytest.mark.parametrize("prec", [E.ECG_FP32, E.ECG_BF16X3])
 def test_mixed_triples_against_oracle(cfg, prec):
-    """(T) and Lambda-(T) with FP32 or BF16 hi/lo products (FP64 combine and sums).
+    """(T) and Lambda-(T) with FP32 or BF16 hi/lo products on packed planes (K stored in
+    FP32, the combine's sums in FP64).
 
     Per product term the FP32 path is accurate to ~2^-24 and the BF16 hi/lo path
     to ~2^-16 relative (the dropped lo*lo term); E[T] is quadratic in K, so the
@@ -94,22 +95,23 @@ def test_mixed_triples_against_oracle(cfg, prec):
     assert abs(lt - lt64) < tol * abs(lt64) and abs(ltb - ltb64) < tol * abs(ltb64), (lt, lt64, ltb, ltb64)
 
 
-def test_triples_blocks_are_tuned_and_do_not_change_results():
-    """BF16x3 (T) in blocks of consecutive triplets: the first call of >= 64 triplets measures
-    the block size (1, 2, 4 or 8) and keeps it.  Members are reduced in triplet order, so a
-    second call gives bitwise the same energies; disjoint ranges add up to the whole (1e-12
-    relative); the result is within the BF16 hi/lo bar (4e-5) of FP64."""
+@pytest.mark.parametrize("prec,tol", [(E.ECG_BF16X3, 4e-5), (E.ECG_FP32, 1e-6)])
+def test_triples_blocks_are_tuned_and_do_not_change_results(prec, tol):
+    """Packed (T) (BF16 hi/lo or FP32 products) in blocks of consecutive triplets: the first
+    call of >= 64 triplets measures the block size (1, 2, 4 or 8) and keeps it.  Members are
+    reduced in triplet order, so a second call gives bitwise the same energies; disjoint ranges
+    add up to the whole (1e-12 relative); the result is within the precision's bar of FP64."""
     cfg = (8, 30, 60, 3)
     EC = E.ECInfo.synthetic(*cfg, scale=0.7 * conditioned_scale(*cfg[:3]))
     T1, T2, _ = EC.ctx.mp2()
     et64, etb64 = EC.ctx.pert_triples(T1, T2)
-    e1 = EC.ctx.pert_triples(T1, T2, precision=E.ECG_BF16X3)
+    e1 = EC.ctx.pert_triples(T1, T2, precision=prec)
     nb = EC.ctx.stats()["triples_block"]
     assert nb in (1, 2, 4, 8)
-    e2 = EC.ctx.pert_triples(T1, T2, precision=E.ECG_BF16X3)
+    e2 = EC.ctx.pert_triples(T1, T2, precision=prec)
     assert e1 == e2
-    assert abs(e1[0] - et64) < 4e-5 * abs(et64) and abs(e1[1] - etb64) < 4e-5 * abs(etb64)
+    assert abs(e1[0] - et64) < tol * abs(et64) and abs(e1[1] - etb64) < tol * abs(etb64)
     n = 8 * 9 * 10 // 6 - 8
-    a = EC.ctx.pert_triples_range(T1, T2, 0, 37, precision=E.ECG_BF16X3)
-    b = EC.ctx.pert_triples_range(T1, T2, 37, n - 37, precision=E.ECG_BF16X3)
+    a = EC.ctx.pert_triples_range(T1, T2, 0, 37, precision=prec)
+    b = EC.ctx.pert_triples_range(T1, T2, 37, n - 37, precision=prec)
     assert abs(a[1] + b[1] - e1[1]) < 1e-12 * abs(e1[1])
