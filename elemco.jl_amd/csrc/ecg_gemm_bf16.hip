@@ -393,6 +393,14 @@ __device__ __forceinline__ void dma16(const void* src, lds_vptr dst) {
   asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory");
 }
 
+// 16 bytes per lane from global memory at base + off (base uniform, in SGPRs; off a
+// per-lane byte offset) into LDS at M0 + 16 * lane; otherwise as dma16.
+__device__ __forceinline__ void dma16s(const void* base, unsigned off, lds_vptr dst) {
+  const unsigned m0 = (unsigned)(__UINTPTR_TYPE__)dst;
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(m0), "v"(off), "s"(base)
+               : "memory");
+}
+
 // The same product with the operand slabs moved HBM/L2 -> LDS by the load unit
 // (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at M0 + 16 l), three
 // LDS buffers in rotation: slab s+2 is in flight while slab s is multiplied, with
@@ -417,23 +425,31 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
   const int kend = min(p.kpad, kbeg + p.kchunk);
 
   // slab k0 -> one buffer: A is 32 blocks of 16 rows x 64 bytes (2 planes x 16), B 16 blocks;
-  // wave w moves blocks w, w+8, ... (4 of A, 2 of B: six 1 KB loads per wave)
-  const int lrow = lane >> 2, lslot = lane & 3;
+  // wave w moves blocks w, w+8, ... (4 of A, 2 of B: six 1 KB loads per wave).  Lane l of a
+  // block loads row 16 c + l/4 into slot l%4, i.e. chunk (l%4) ^ ((l/8)%4) of that row: a
+  // uniform block base plus a per-lane byte offset that is the same for every block of an
+  // operand.
+  // With several K segments (MS) the segment's planes and strides change along K; the
+  // per-lane 64-bit addresses then cost fewer scalar registers than uniform bases of every
+  // block (which made the compiler spill scalar registers inside the loop).
+  const int lrow = lane >> 2, lch = (lane & 3) ^ ((lrow >> 1) & 3);
   auto issue = [&](u32x4_t (&As)[2][XM][4], u32x4_t (&Bs)[2][XN][4], int k0) {
     const SlabSrc g = slab_src<MS>(p, k0, zb);
+    const unsigned offa = (unsigned)(lrow * g.lda + 8 * lch) * 2u;
+    const unsigned offb = (unsigned)(lrow * g.ldb + 8 * lch) * 2u;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int c = wave + 8 * j, pl = c >> 4, row = 16 * (c & 15) + lrow;
-      const int ch = lslot ^ ((row >> 1) & 3);
-      const u16* src = (pl ? g.al : g.ah) + (size_t)(bm + row) * g.lda + g.kl + 8 * ch;
-      dma16(src, (lds_vptr)&As[pl][16 * (c & 15)][0]);
+      const int c = wave + 8 * j, pl = c >> 4, r0 = 16 * (c & 15);
+      const u16* pa = pl ? g.al : g.ah;
+      if constexpr (MS) dma16(pa + (size_t)(bm + r0 + lrow) * g.lda + g.kl + 8 * lch, (lds_vptr)&As[pl][r0][0]);
+      else dma16s(pa + (size_t)(bm + r0) * g.lda + g.kl, offa, (lds_vptr)&As[pl][r0][0]);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int c = wave + 8 * j, pl = c >> 3, row = 16 * (c & 7) + lrow;
-      const int ch = lslot ^ ((row >> 1) & 3);
-      const u16* src = (pl ? g.bl : g.bh) + (size_t)(bn + row) * g.ldb + g.kl + 8 * ch;
-      dma16(src, (lds_vptr)&Bs[pl][16 * (c & 7)][0]);
+      const int c = wave + 8 * j, pl = c >> 3, r0 = 16 * (c & 7);
+      const u16* pb = pl ? g.bl : g.bh;
+      if constexpr (MS) dma16(pb + (size_t)(bn + r0 + lrow) * g.ldb + g.kl + 8 * lch, (lds_vptr)&Bs[pl][r0][0]);
+      else dma16s(pb + (size_t)(bn + r0) * g.ldb + g.kl, offb, (lds_vptr)&Bs[pl][r0][0]);
     }
   };
 
@@ -511,14 +527,6 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
   x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
-}
-
-// 16 bytes per lane from global memory at base + off (base uniform, in SGPRs; off a
-// per-lane byte offset) into LDS at M0 + 16 * lane; otherwise as dma16.
-__device__ __forceinline__ void dma16s(const void* base, unsigned off, lds_vptr dst) {
-  const unsigned m0 = (unsigned)(__UINTPTR_TYPE__)dst;
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(m0), "v"(off), "s"(base)
-               : "memory");
 }
 
 // FP32 planes and row strides (in floats) of the segment holding slab k0: the packed
