@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "ecg_cc.h"
+#include "ecg_dfprep.h"
 #include "ecg_fcidump.h"
 #include "ecg_grad.h"
 #include "ecg_lambda.h"
@@ -224,6 +225,55 @@ int ecg_fcidump_df(ecg_fcidump* f, double rtol, int* naux, double* BL, double* B
     if (BR) std::memcpy(BR, (d.BR.empty() ? d.BL : d.BR).data(), sizeof(double) * d.BL.size());
     return 0;
   });
+}
+
+int ecg_df_ao_to_mo(const double* C, int nao, int nmo, const double* AAP, const double* PQ, int nfit, double tol,
+                    int device, int* naux, double* B) {
+  return host_guarded([&] {
+    ECG_CHECK(C && AAP && PQ && B && naux, "ecg_df_ao_to_mo: NULL argument");
+    ECG_CHECK(nao > 0 && nmo > 0 && nfit > 0, "ecg_df_ao_to_mo: sizes must be positive");
+    int ndev = 0;
+    ECG_HIP(hipGetDeviceCount(&ndev));
+    ECG_CHECK(device >= 0 && device < ndev, "no such HIP device");
+    int r = 0;
+    const std::vector<double> M = ecg::sqrtinvchol(PQ, nfit, tol, r);
+    ecg::df_ao_to_mo_device(device, C, nao, nmo, AAP, nfit, M.data(), r, B);
+    *naux = r;
+    return 0;
+  });
+}
+
+int ecg_create_from_ao(ecg_ctx** out, int nocc, const double* C, int nao, int nmo, const double* h_ao,
+                       const double* AAP, const double* PQ, int nfit, double tol, double e0, int device,
+                       size_t workspace_bytes) {
+  if (!out) return -1;
+  *out = nullptr;
+  std::vector<double> B, h;
+  int naux = 0;
+  int st = host_guarded([&] {
+    ECG_CHECK(h_ao, "ecg_create_from_ao: NULL argument");
+    ECG_CHECK(nocc > 0 && nocc < nmo, "ecg_create_from_ao: nocc must lie between 1 and nmo - 1");
+    B.resize((size_t)nmo * nmo * nfit);
+    return 0;
+  });
+  if (st != 0) return st;
+  st = ecg_df_ao_to_mo(C, nao, nmo, AAP, PQ, nfit, tol, device, &naux, B.data());
+  if (st != 0) return st;
+  // h[p,q] = sum C[mu,p] h_ao[mu,nu] C[nu,q]
+  std::vector<double> t((size_t)nao * nmo, 0.0);
+  for (int q = 0; q < nmo; ++q)
+    for (int nu = 0; nu < nao; ++nu) {
+      const double c = C[nu + (size_t)nao * q];
+      for (int mu = 0; mu < nao; ++mu) t[mu + (size_t)nao * q] += h_ao[mu + (size_t)nao * nu] * c;
+    }
+  h.assign((size_t)nmo * nmo, 0.0);
+  for (int q = 0; q < nmo; ++q)
+    for (int p = 0; p < nmo; ++p) {
+      double s = 0.0;
+      for (int mu = 0; mu < nao; ++mu) s += C[mu + (size_t)nao * p] * t[mu + (size_t)nao * q];
+      h[p + (size_t)nmo * q] = s;
+    }
+  return ecg_create(out, nocc, nmo - nocc, naux, h.data(), B.data(), nullptr, e0, device, workspace_bytes);
 }
 
 int ecg_dims(const ecg_ctx* ctx, int* nocc, int* nvirt, int* naux) {

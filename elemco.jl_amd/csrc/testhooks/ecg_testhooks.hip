@@ -1,9 +1,11 @@
 // Test-only entry points (libecg_testhooks.so): expose the tensor engine so the
 // GPU tests can check single contractions and permutations against numpy.
 // Not part of the product C ABI.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
+#include "../ecg_dfprep.h"
 #include "../ecg_tensor.h"
 
 namespace {
@@ -110,6 +112,19 @@ double ecgt_bench_contract(int prec, int na, const long* adims, const char* la, 
     return ms / reps;
   } catch (const std::exception&) {
     return -1.0;
+  }
+}
+
+// sqrtinvchol of the DF preparation (host only): M (n x n, first *rank columns used).
+int ecgt_sqrtinvchol(const double* A, int n, double tol, int* rank, double* M) {
+  try {
+    int r = 0;
+    const std::vector<double> m = ecg::sqrtinvchol(A, n, tol, r);
+    std::copy(m.begin(), m.end(), M);
+    *rank = r;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
   }
 }
 

@@ -68,6 +68,10 @@ _SIGS = {
     "ecg_create_from_fcidump": [ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_size_t],
     "ecg_dims": [_vp, _ip, _ip, _ip],
+    "ecg_df_ao_to_mo": [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
+                        _dp],
+    "ecg_create_from_ao": [ctypes.POINTER(_vp), ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                           ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_size_t],
 }
 _RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p}
 EXPORTED = tuple(_SIGS)
@@ -113,6 +117,25 @@ def fortran(a, shape=None):
     if shape is not None and a.shape != tuple(shape):
         raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
     return np.asfortranarray(a)
+
+
+def df_ao_to_mo(C, AAP, PQ, tol=1e-8, device=0):
+    """MO DF factors ``B[p,q,L]`` from AO three-index integrals ``(mu nu|P)`` and the metric
+    ``(P|Q)`` (``ecg_df_ao_to_mo``: ``generate_3idx_integrals`` with ``sqrtinvchol``,
+    ``src/integrals/dftools.jl:92-125``)."""
+    lib = load()
+    C = fortran(C)
+    nao, nmo = C.shape
+    nfit = PQ.shape[0]
+    AAP = fortran(AAP, (nao, nao, nfit))
+    PQ = fortran(PQ, (nfit, nfit))
+    B = np.zeros((nmo, nmo, nfit), order="F")
+    naux = ctypes.c_int()
+    st = lib.ecg_df_ao_to_mo(_ptr(C), nao, nmo, _ptr(AAP), _ptr(PQ), nfit, float(tol), int(device),
+                             ctypes.byref(naux), _ptr(B))
+    if st != 0:
+        raise ElemcoError(f"ecg_df_ao_to_mo failed ({st}): {lib.ecg_last_error(None).decode()}")
+    return np.asfortranarray(B[:, :, :naux.value])
 
 
 class Fcidump:
@@ -201,6 +224,29 @@ class Context:
                                          int(workspace_bytes))
         if st != 0:
             raise ElemcoError(f"ecg_create_from_fcidump failed ({st}): {lib.ecg_last_error(None).decode()}")
+        self = cls.__new__(cls)
+        self._h, self._lib = handle, lib
+        o, v, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(lib.ecg_dims(handle, ctypes.byref(o), ctypes.byref(v), ctypes.byref(q)))
+        self.nocc, self.nvirt, self.naux, self.norb = o.value, v.value, q.value, o.value + v.value
+        return self
+
+    @classmethod
+    def from_ao(cls, nocc, C, h_ao, AAP, PQ, tol=1e-8, e0=0.0, device=0, workspace_bytes=0):
+        """Context from AO integrals (``ecg_create_from_ao``: DF preparation on the device,
+        ``h = C^T h_AO C``)."""
+        lib = load()
+        C = fortran(C)
+        nao, nmo = C.shape
+        nfit = PQ.shape[0]
+        h_ao = fortran(h_ao, (nao, nao))
+        AAP = fortran(AAP, (nao, nao, nfit))
+        PQ = fortran(PQ, (nfit, nfit))
+        handle = _vp()
+        st = lib.ecg_create_from_ao(ctypes.byref(handle), int(nocc), _ptr(C), nao, nmo, _ptr(h_ao), _ptr(AAP), _ptr(PQ),
+                                    nfit, float(tol), float(e0), int(device), int(workspace_bytes))
+        if st != 0:
+            raise ElemcoError(f"ecg_create_from_ao failed ({st}): {lib.ecg_last_error(None).decode()}")
         self = cls.__new__(cls)
         self._h, self._lib = handle, lib
         o, v, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -114,6 +114,31 @@ function ECGContext(fcidump::AbstractString; rtol=0.0, device=0)
   return ctx
 end
 
+"""
+    ECGContext(nocc, C, h_ao, AAP, PQ; tol=1e-8, e0=0.0, device=0)
+
+DF set-up from AO integrals (`ecg_create_from_ao`): `B[p,q,L]` of
+`generate_3idx_integrals` (src/integrals/dftools.jl:92-125) with `sqrtinvchol` of the
+metric `PQ` (src/tools/tensortools.jl:377-391), `h = C' h_ao C`; `C` is nao×nmo, `AAP`
+nao×nao×nfit (the reference's `AAP` before the metric), `PQ` nfit×nfit.
+"""
+function ECGContext(nocc::Integer, C::Matrix{Float64}, h_ao::Matrix{Float64}, AAP::Array{Float64,3},
+                    PQ::Matrix{Float64}; tol=1e-8, e0=0.0, device=0)
+  nao, nmo = size(C)
+  nfit = size(PQ, 1)
+  ref = Ref{Ptr{Cvoid}}(C_NULL)
+  st = ccall((:ecg_create_from_ao, LIB), Cint,
+             (Ref{Ptr{Cvoid}}, Cint, Ptr{Cdouble}, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cint,
+              Cdouble, Cdouble, Cint, Csize_t),
+             ref, nocc, C, nao, nmo, h_ao, AAP, PQ, nfit, tol, e0, device, 0)
+  _check(nothing, st)
+  o, v, q = Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0)
+  ccall((:ecg_dims, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}, Ref{Cint}), ref[], o, v, q)
+  ctx = ECGContext(ref[], o[], v[], q[])
+  finalizer(c -> (c.handle != C_NULL && ccall((:ecg_destroy, LIB), Cint, (Ptr{Cvoid},), c.handle); c.handle = C_NULL), ctx)
+  return ctx
+end
+
 _z1(ctx) = zeros(ctx.nvirt, ctx.nocc)
 _z2(ctx) = zeros(ctx.nvirt, ctx.nvirt, ctx.nocc, ctx.nocc)
 

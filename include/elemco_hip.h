@@ -90,6 +90,21 @@ int ecg_fcidump_df(ecg_fcidump* fd, double rtol, int* naux, double* BL, double* 
 /* ecg_fcidump_read + ecg_fcidump_df + ecg_create with nocc = NELEC / 2: the
  * ccdriver(EC, method; fcidump) set-up (src/cc/ccdriver.jl:37-76) in one call. */
 int ecg_create_from_fcidump(ecg_ctx** ctx, const char* path, double rtol, int device, size_t workspace_bytes);
+/* DF preparation in front of the path: MO factors from AO three-index integrals, as
+ * generate_AO_DF_integrals / generate_3idx_integrals (src/integrals/dftools.jl:28-125)
+ * after the integral library has produced them.  C: nao x nmo MO coefficients,
+ * AAP: (mu nu|P) nao x nao x nfit, PQ: metric (P|Q) nfit x nfit.  M = sqrtinvchol(PQ, tol)
+ * (src/tools/tensortools.jl:377-391: pivoted Cholesky stopped at pivots <= tol, then
+ * U^{-1}, or the minimum-norm solution when the metric is rank-deficient; host side),
+ * B[p,q,L] = sum C[mu,p] (mu nu|P) M[P,L] C[nu,q] by FP64 MFMA contractions on `device`.
+ * B must hold nmo*nmo*nfit doubles; the first *naux = rank slices are written. */
+int ecg_df_ao_to_mo(const double* C, int nao, int nmo, const double* AAP, const double* PQ, int nfit, double tol,
+                    int device, int* naux, double* B);
+/* ecg_df_ao_to_mo + h_MO = C^T h_AO C + ecg_create: the DF set-up of a CC calculation
+ * (`@dfhf; @cc`, examples/h2o.jl; dftools.jl:92-125) on the GPU path. */
+int ecg_create_from_ao(ecg_ctx** ctx, int nocc, const double* C, int nao, int nmo, const double* h_ao,
+                       const double* AAP, const double* PQ, int nfit, double tol, double e0, int device,
+                       size_t workspace_bytes);
 /* Orbital-space sizes of a context (the reference's EC.space 'o', 'v' and the DF
  * basis, src/infos/ecinfos.jl:231-244); any output may be NULL. */
 int ecg_dims(const ecg_ctx* ctx, int* nocc, int* nvirt, int* naux);

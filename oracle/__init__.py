@@ -15,6 +15,8 @@ the ElemCo.jl v0.14.0 source tree):
 * ``src/solvers/diis.jl``         Pulay DIIS                        -> ccsd.py
 * ``src/cc/cc_triples.jl``        (T) and Lambda-(T)                -> triples.py
 * ``src/cc/cc_lagrange.jl``       Lambda residual, 1-RDM, Lambda loop -> lagrange.py
+* ``src/integrals/dftools.jl``,   DF preparation (AO -> MO factors,
+  ``src/tools/tensortools.jl``    sqrtinvchol of the metric)        -> dfprep.py
 
 and generates the seeded synthetic density-fitted systems (synthetic.py).
 
@@ -26,5 +28,7 @@ Parity pins (tests in ``tests/test_oracle.py``):
   - brute-force [T] from explicit T3 (the reference's ``cc_tests.jl:54-75``);
   - finite differences of the CC Lagrangian for the Lambda residual and of the
     energy for the 1-RDM;
-  - splitmix64 published vectors for the synthetic generator.
+  - splitmix64 published vectors for the synthetic generator;
+  - the DF preparation against its closed form C^T C^T (mu nu|P) pinv((P|Q)) (P|ls) C C
+    and M M^T = pinv((P|Q)), full-rank and rank-deficient metrics.
 """

@@ -223,3 +223,26 @@ def test_fcidump_roundtrip(tmp_path):
     edf = ccdriver(ECInfo.from_df(s), "ccsd(t)")
     assert abs(e4["HF"] - edf["HF"]) < 1e-12
     assert abs(e4["CCSD(T)"] - edf["CCSD(T)"]) < 1e-11
+
+
+@pytest.mark.parametrize("rank", [50, 13])
+def test_dfprep_oracle_closed_form(rank):
+    """oracle.dfprep (generate_3idx_integrals + sqrtinvchol, dftools.jl:92-125,
+    tensortools.jl:377-391): sum_L mmL mmL = C^T C^T (mu nu|P) pinv((P|Q)) (P|ls) C C, for a
+    full-rank and a rank-deficient metric; M M^T = pinv((P|Q))."""
+    from oracle.dfprep import df_ao_to_mo, sqrtinvchol
+    rng = np.random.default_rng(rank)
+    nao, nmo, nfit = 8, 6, 20
+    G = rng.standard_normal((nfit, rank))
+    PQ = G @ G.T
+    AAP = rng.standard_normal((nao, nao, nfit))
+    AAP = AAP + AAP.transpose(1, 0, 2)
+    C = rng.standard_normal((nao, nmo))
+    Pinv = np.linalg.pinv(PQ, rcond=1e-12)
+    M = sqrtinvchol(PQ, 1e-9)
+    assert M.shape == (nfit, min(rank, nfit))
+    assert np.max(np.abs(M @ M.T - Pinv)) < 1e-9 * np.max(np.abs(Pinv))
+    B = df_ao_to_mo(C, AAP, PQ, 1e-9)
+    ref = np.einsum("mp,nq,mnP,PQ,lsQ,lr,st->pqrt", C, C, AAP, Pinv, AAP, C, C, optimize=True)
+    assert np.max(np.abs(np.einsum("pqL,rsL->pqrs", B, B) - ref)) < 1e-9 * np.max(np.abs(ref))
+
