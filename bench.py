@@ -30,8 +30,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample=(16, 144, 480, 7)):
-    """Time the CPU oracle (numpy/OpenBLAS restatement) on one CCSD residual of a bounded sample."""
+def cpu_baseline(sample=(20, 180, 600, 1)):
+    """Time the CPU oracle (numpy/OpenBLAS restatement) on one CCSD residual of a bounded sample:
+    the o=20 v=180 naux=600 system (one residual is about 10 s of host work; the o=40 v=360
+    residual would take minutes)."""
     sys.path.insert(0, ROOT)
     from oracle.ccsd import ECInfo, calc_fock_matrix, calc_MP2, calc_cc_resid
     from oracle.synthetic import make_df_system
