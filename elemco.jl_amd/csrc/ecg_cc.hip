@@ -55,20 +55,40 @@ __global__ void update_singles_kernel(int nv, int no, const double* __restrict__
 }
 
 // Per-(i,j) block sums  d = sum_ab X[a,b,i,j] Y[a,b,i,j],  e = sum_ab X[b,a,i,j] Y[a,b,i,j].
+// The v x v block is walked in 32 x 32 tiles: X and Y of tile (A, B) and X of the mirrored
+// tile (B, A) are read along their unit-stride index (coalesced), the mirrored tile is
+// transposed through LDS.  Each thread keeps 4 elements of a tile (256 threads).
 __global__ __launch_bounds__(256) void cs_dots_kernel(int nv, const double* __restrict__ X,
                                                       const double* __restrict__ Y, double* part) {
+  __shared__ double xt[32][33];   // mirrored tile, [b][a] as read
   __shared__ double sd[4], se[4];
-  long off = (long)blockIdx.x * nv * nv;
+  const long off = (long)blockIdx.x * nv * nv;
   const double* x = X + off;
   const double* y = Y + off;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+  const int nt = (nv + 31) / 32;
   double d = 0.0, e = 0.0;
-  int vv = nv * nv;
-  for (int t = threadIdx.x; t < vv; t += blockDim.x) {
-    int a = t % nv, b = t / nv;
-    double yv = y[t];
-    d += x[t] * yv;
-    e += x[b + (long)a * nv] * yv;
-  }
+  for (int tb = 0; tb < nt; ++tb)
+    for (int ta = 0; ta < nt; ++ta) {
+      // mirrored tile: rows b in tile tb (unit stride), columns a in tile ta
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int b = tb * 32 + tx, a = ta * 32 + ty + 8 * k;
+        xt[ty + 8 * k][tx] = (a < nv && b < nv) ? x[b + (long)a * nv] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int a = ta * 32 + tx, b = tb * 32 + ty + 8 * k;
+        if (a < nv && b < nv) {
+          const long t = a + (long)b * nv;
+          const double yv = y[t];
+          d += x[t] * yv;
+          e += xt[tx][ty + 8 * k] * yv;   // X[b, a]
+        }
+      }
+    }
   for (int s = 32; s > 0; s >>= 1) {
     d += __shfl_down(d, s, 64);
     e += __shfl_down(e, s, 64);
