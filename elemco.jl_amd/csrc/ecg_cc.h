@@ -106,8 +106,8 @@ struct Context {
     Prec pr = Prec::F64;
     const double *t1 = nullptr, *t2 = nullptr;
     double *r1 = nullptr, *r2 = nullptr;
-    const void *ws = nullptr, *pk = nullptr, *arena = nullptr;
-    size_t wsb = 0, pkb = 0;
+    const void *ws = nullptr, *pk = nullptr, *keep = nullptr, *arena = nullptr;
+    size_t wsb = 0, pkb = 0, keepb = 0;
     hipGraphExec_t exec = nullptr;
   };
   std::vector<ResidualGraph> rgraphs;

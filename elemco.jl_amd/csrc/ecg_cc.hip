@@ -318,6 +318,8 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
   long ab = std::max<long>(1, std::min<long>(v, (long)(free_b * 0.8 / sizeof(double)) / per_a));
   double* V = ops.arena.take(ab * per_a);
   ops.gemm.stream = stream;
+  // T2x is read-only here: its packed BF16 planes serve every block's second product
+  ops.gemm.drop_kept_planes();
   for (long a0 = 0; a0 < v; a0 += ab) {
     const long na = std::min<long>(ab, v - a0);
     const long nb = v - a0;
@@ -343,9 +345,11 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
       h.C = R2r + a0 + (a0 + b0) * v; h.c_ms0 = 1; h.c_md0 = (int)na; h.c_ms1 = v;
       h.c_ns0 = v * v; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
       h.alpha = part == 0 ? 0.5 : 1.0; h.beta = 1.0;
+      h.b_keep = true;
       ops.gemm.run_prec(h, pr);
     }
   }
+  ops.gemm.drop_kept_planes();
   ops.arena.release_to(mark);
 }
 
@@ -491,6 +495,7 @@ void Context::residual_graph(int method, Prec pr, const double* T1d, const doubl
   auto scratch = [&](ResidualGraph& g) {
     g.ws = ops.gemm.ws.p; g.wsb = ops.gemm.ws.bytes;
     g.pk = ops.gemm.pk.p; g.pkb = ops.gemm.pk.bytes;
+    g.keep = ops.gemm.pk_keep.p; g.keepb = ops.gemm.pk_keep.bytes;
     g.arena = ops.arena.buf.p;
   };
   for (size_t k = 0; k < rgraphs.size(); ++k) {
@@ -498,7 +503,8 @@ void Context::residual_graph(int method, Prec pr, const double* T1d, const doubl
     if (g.method != method || g.pr != pr || g.t1 != T1d || g.t2 != T2d || g.r1 != R1d || g.r2 != R2d) continue;
     ResidualGraph now;
     scratch(now);
-    if (now.ws == g.ws && now.wsb == g.wsb && now.pk == g.pk && now.pkb == g.pkb && now.arena == g.arena) {
+    if (now.ws == g.ws && now.wsb == g.wsb && now.pk == g.pk && now.pkb == g.pkb && now.keep == g.keep &&
+        now.keepb == g.keepb && now.arena == g.arena) {
       residual_calls++;
       ECG_HIP(hipGraphLaunch(g.exec, stream));
       return;
@@ -526,7 +532,7 @@ void Context::residual_graph(int method, Prec pr, const double* T1d, const doubl
   ECG_HIP(hipStreamEndCapture(stream, &graph));
   ResidualGraph after;
   scratch(after);
-  ECG_CHECK(after.ws == g.ws && after.pk == g.pk && after.arena == g.arena,
+  ECG_CHECK(after.ws == g.ws && after.pk == g.pk && after.keep == g.keep && after.arena == g.arena,
             "residual graph: a scratch buffer was reallocated during capture");
   hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);

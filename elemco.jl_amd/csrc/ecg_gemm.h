@@ -17,6 +17,8 @@
 #pragma once
 #include "ecg_common.h"
 
+#include <vector>
+
 namespace ecg {
 
 constexpr int MAXSEG = 4;
@@ -36,6 +38,9 @@ struct GemmOp {
   double alpha = 1.0, beta = 0.0;
   int nseg = 0;            // 0: single product (A, a_k, B, b_k, K); else seg[0..nseg)
   GemmSeg seg[MAXSEG];
+  // BF16 hi/lo products: B's data does not change until GemmEngine::drop_kept_planes(), so
+  // its packed planes are kept and reused by the next product with the same B operand
+  bool b_keep = false;
 
   // Plain column-major helpers: A is M x K (lda), or K x M when transposed.
   GemmOp& a_colmajor(const double* p, long lda, bool trans) {
@@ -114,6 +119,11 @@ struct GemmEngine {
   // straight into LDS (gemm_bf16x3_dma_kernel)
   int bf16_kernel = 1;
   void run_bf16x3(const GemmOp& op);
+  // packed B planes kept for products with GemmOp::b_keep (one operand, identified by its
+  // addressing); drop_kept_planes() forgets them (the caller's B may change afterwards)
+  DevBuf pk_keep;
+  std::vector<long> keep_key;
+  void drop_kept_planes() { keep_key.clear(); }
   // Pack rows r < rows of src (row offset split as in GemmOp: (r % d0)*s0 + (r / d0)*s1,
   // element k at + k*sk, k < K) into hi/lo planes [r][kpad], kpad a multiple of 32 >= K.
   // Rows are written up to the next multiple of 64.

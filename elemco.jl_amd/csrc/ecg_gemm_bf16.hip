@@ -690,17 +690,41 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   bytes += 8.0 * ((double)(op.M + op.N) * ktot + (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N);
   const int tiles_m = (op.M + XM - 1) / XM, tiles_n = (op.N + XN - 1) / XN;
   const long Mp = (long)tiles_m * XM, Np = (long)tiles_n * XN;
-  pk.ensure(sizeof(u16) * 2 * (size_t)(Mp + Np) * kpad);
-  u16* Ah = static_cast<u16*>(pk.p);
-  u16* Al = Ah + (size_t)Mp * kpad;
-  u16* Bh = Al + (size_t)Mp * kpad;
-  u16* Bl = Bh + (size_t)Np * kpad;
+  u16 *Ah, *Al, *Bh, *Bl;
+  bool pack_b = true;
+  if (op.b_keep) {
+    // B planes kept between products: reuse them when B's addressing is the kept one
+    std::vector<long> key{nseg, op.N, kpad, op.b_s0, op.b_s1, op.b_d0};
+    for (int s = 0; s < nseg; ++s) {
+      key.push_back((long)(__UINTPTR_TYPE__)sg[s].B);
+      key.push_back(sg[s].b_k);
+      key.push_back(sg[s].K);
+    }
+    if (key == keep_key) {
+      pack_b = false;
+    } else {
+      keep_key.clear();   // invalid while the planes are being replaced
+      pk_keep.ensure(sizeof(u16) * 2 * (size_t)Np * kpad);
+      keep_key = key;
+    }
+    pk.ensure(sizeof(u16) * 2 * (size_t)Mp * kpad);
+    Ah = static_cast<u16*>(pk.p);
+    Al = Ah + (size_t)Mp * kpad;
+    Bh = static_cast<u16*>(pk_keep.p);
+    Bl = Bh + (size_t)Np * kpad;
+  } else {
+    pk.ensure(sizeof(u16) * 2 * (size_t)(Mp + Np) * kpad);
+    Ah = static_cast<u16*>(pk.p);
+    Al = Ah + (size_t)Mp * kpad;
+    Bh = Al + (size_t)Mp * kpad;
+    Bl = Bh + (size_t)Np * kpad;
+  }
   pa.rows = op.M; pa.kpad = kpad; pa.s0 = op.a_s0; pa.s1 = op.a_s1; pa.d0 = op.a_d0; pa.hi = Ah; pa.lo = Al;
   pb.rows = op.N; pb.kpad = kpad; pb.s0 = op.b_s0; pb.s1 = op.b_s1; pb.d0 = op.b_d0; pb.hi = Bh; pb.lo = Bl;
   const long nba = (Mp / PR) * (long)(kpad / XK), nbb = (Np / PR) * (long)(kpad / XK);
   ECG_CHECK(nba < (1L << 31) && nbb < (1L << 31), "gemm: operand too large for the BF16 packing grid");
   hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
-  hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
+  if (pack_b) hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
   ECG_HIP(hipGetLastError());
 
   X3Params x{};
