@@ -119,6 +119,10 @@ struct GemmEngine {
   // straight into LDS (gemm_bf16x3_dma_kernel)
   int bf16_kernel = 1;
   void run_bf16x3(const GemmOp& op);
+  // FP32 products on FP32 planes packed on the fly (ecg_gemm_bf16.hip, the (T)'s FP32 kernel)
+  void run_f32p(const GemmOp& op);
+  void run_planes(const GemmOp& op, bool f32);
+  int f32_kernel = 1;   // large FP32 products: 0 generic FP32 MFMA kernel, 1 packed FP32 planes
   // packed B planes kept for products with GemmOp::b_keep (one operand, identified by its
   // addressing); drop_kept_planes() forgets them (the caller's B may change afterwards)
   DevBuf pk_keep;

@@ -298,6 +298,13 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     }
     pr = Prec::F32;
   }
+  if (pr == Prec::F32 && f32_kernel == 1 && op.batch == 1 &&
+      (double)op.M * op.N / ((double)op.M + op.N) >= 256.0) {
+    // large FP32 products: operands packed into FP32 planes (12 bytes moved per element)
+    // for the direct-to-LDS FP32 kernel
+    run_f32p(op);
+    return;
+  }
   ECG_CHECK(pr == Prec::F64 || pr == Prec::F32, "gemm: unknown precision");
   GemmParams p{};
   GemmSeg one;

@@ -103,15 +103,18 @@ __global__ __launch_bounds__(256) void pack_bf16x3_kernel(PackArgs p, int nrt) {
 }
 
 // FP32 plane with the slab permutation (position 8*(k%4) + k/4 of each 32-wide slab holds
-// k); grid and reads as pack_bf16x3_kernel.
+// k); grid, reads and K segments as pack_bf16x3_kernel.
 __global__ __launch_bounds__(256) void pack_f32p_kernel(PackArgs p, int nrt) {
   __shared__ float fs[XK][PR + 1];
   const int t = threadIdx.x;
   const long bid = blockIdx.x;
   const int r0 = (int)(bid % nrt) * PR, k0 = (int)(bid / nrt) * XK;
   const double* src = p.src[0];
-  const long sk = p.sk[0];
-  const int klen = p.sK[0];
+  long sk = p.sk[0];
+  int kl0 = k0, klen = p.sK[0];
+#pragma unroll
+  for (int q = 1; q < MAXSEG; ++q)
+    if (q < p.nseg && k0 >= p.kp[q]) { src = p.src[q]; sk = p.sk[q]; kl0 = k0 - p.kp[q]; klen = p.sK[q]; }
   const int row = t & (PR - 1);
   const int r = r0 + row;
   const bool rok = r < p.rows;
@@ -119,7 +122,7 @@ __global__ __launch_bounds__(256) void pack_f32p_kernel(PackArgs p, int nrt) {
 #pragma unroll
   for (int j = 0; j < XK / 4; ++j) {
     const int kk = (t >> 6) + 4 * j;
-    const int k = k0 + kk;
+    const int k = kl0 + kk;
     const bool ok = rok && k < klen;
     const double x = src[ok ? roff + (long)k * sk : 0];
     fs[kk][row] = ok ? (float)x : 0.f;
@@ -662,8 +665,14 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
 
 }  // namespace
 
-void GemmEngine::run_bf16x3(const GemmOp& op) {
-  ECG_CHECK(op.batch == 1, "gemm: batched BF16 hi/lo products are not supported");
+void GemmEngine::run_bf16x3(const GemmOp& op) { run_planes(op, false); }
+
+void GemmEngine::run_f32p(const GemmOp& op) { run_planes(op, true); }
+
+// Operands packed on the fly (BF16 hi/lo planes, or one FP32 plane for f32), then the
+// packed-plane kernel; split K when the tiles alone cannot fill the GPU.
+void GemmEngine::run_planes(const GemmOp& op, bool f32) {
+  ECG_CHECK(op.batch == 1, "gemm: batched packed products are not supported");
   GemmSeg one;
   one.A = op.A; one.a_k = op.a_k; one.B = op.B; one.b_k = op.b_k; one.K = op.K;
   const GemmSeg* sg = op.nseg > 0 ? op.seg : &one;
@@ -694,7 +703,7 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   bool pack_b = true;
   if (op.b_keep) {
     // B planes kept between products: reuse them when B's addressing is the kept one
-    std::vector<long> key{nseg, op.N, kpad, op.b_s0, op.b_s1, op.b_d0};
+    std::vector<long> key{f32 ? 1L : 0L, nseg, op.N, kpad, op.b_s0, op.b_s1, op.b_d0};
     for (int s = 0; s < nseg; ++s) {
       key.push_back((long)(__UINTPTR_TYPE__)sg[s].B);
       key.push_back(sg[s].b_k);
@@ -719,12 +728,21 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
     Bh = Al + (size_t)Mp * kpad;
     Bl = Bh + (size_t)Np * kpad;
   }
+  // an FP32 plane takes the bytes of the two BF16 planes (4 per element): its rows start at
+  // the hi plane's address with a row length of kpad floats
   pa.rows = op.M; pa.kpad = kpad; pa.s0 = op.a_s0; pa.s1 = op.a_s1; pa.d0 = op.a_d0; pa.hi = Ah; pa.lo = Al;
   pb.rows = op.N; pb.kpad = kpad; pb.s0 = op.b_s0; pb.s1 = op.b_s1; pb.d0 = op.b_d0; pb.hi = Bh; pb.lo = Bl;
+  pa.f32 = reinterpret_cast<float*>(Ah);
+  pb.f32 = reinterpret_cast<float*>(Bh);
   const long nba = (Mp / PR) * (long)(kpad / XK), nbb = (Np / PR) * (long)(kpad / XK);
-  ECG_CHECK(nba < (1L << 31) && nbb < (1L << 31), "gemm: operand too large for the BF16 packing grid");
-  hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
-  if (pack_b) hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
+  ECG_CHECK(nba < (1L << 31) && nbb < (1L << 31), "gemm: operand too large for the packing grid");
+  if (f32) {
+    hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+    if (pack_b) hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
+  } else {
+    hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+    if (pack_b) hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
+  }
   ECG_HIP(hipGetLastError());
 
   X3Params x{};
@@ -759,7 +777,9 @@ void GemmEngine::run_bf16x3(const GemmOp& op) {
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
   const dim3 grid((unsigned)tiles, 1, (unsigned)ksplit);
-  if (bf16_kernel == 1)
+  if (f32)
+    hipLaunchKernelGGL((gemm_f32p_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
+  else if (bf16_kernel == 1)
     hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
   else
     hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), grid, dim3(512), 0, stream, x);
@@ -791,6 +811,8 @@ void GemmEngine::pack_rows_f32(const double* src, long rows, int d0, long s0, lo
   PackArgs pa{};
   pa.rows = (int)rows; pa.kpad = kpad; pa.nseg = 1;
   for (int s = 0; s < MAXSEG; ++s) { pa.src[s] = src; pa.sk[s] = sk; pa.sK[s] = K; }
+  pa.kp[0] = 0;
+  for (int s = 1; s <= MAXSEG; ++s) pa.kp[s] = kpad;
   pa.s0 = s0; pa.s1 = s1; pa.d0 = d0; pa.f32 = out;
   hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)(rt * (kpad / XK))), dim3(256), 0, stream, pa, (int)rt);
   ECG_HIP(hipGetLastError());

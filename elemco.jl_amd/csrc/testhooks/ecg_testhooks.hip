@@ -128,6 +128,15 @@ int ecgt_sqrtinvchol(const double* A, int n, double tol, int* rank, double* M) {
   }
 }
 
+// Selects the FP32 GEMM kernel for large products of the hooks' engine
+// (GemmEngine::f32_kernel: 0 generic, 1 packed FP32 planes); returns the previous choice.
+int ecgt_set_f32_kernel(int k) {
+  Env& e = env();
+  const int prev = e.ops.gemm.f32_kernel;
+  e.ops.gemm.f32_kernel = k;
+  return prev;
+}
+
 // Selects the BF16 hi/lo GEMM kernel of the hooks' engine (GemmEngine::bf16_kernel);
 // returns the previous choice.
 int ecgt_set_bf16_kernel(int k) {

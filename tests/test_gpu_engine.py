@@ -118,3 +118,26 @@ def test_contract_bf16x3_products(kernel, case):
     err = float(np.max(np.abs(got - ref)))
     assert err < bound
     assert err > 0.0          # the path is not silently FP64
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("case", range(len(BF16_CASES)))
+def test_contract_fp32_kernels(kernel, case):
+    """Large FP32 products through both FP32 kernels (generic FP32 MFMA; operands packed into
+    FP32 planes for the direct-to-LDS kernel, K from one slab to many with split K): error
+    bounded by 4 sqrt(K) 2^-23 (|A||B|), and the packed kernel's result is not FP64's."""
+    ext, la, lb, lc = BF16_CASES[case]
+    rng = np.random.default_rng(23)
+    A, B, C = _rand(la, ext, rng), _rand(lb, ext, rng), _rand(lc, ext, rng)
+    ref = np.einsum(f"{la},{lb}->{lc}", A, B) + C
+    prev = hooks().ecgt_set_f32_kernel(kernel)
+    try:
+        got = gpu_contract(1.0, A, la, B, lb, 1.0, C, lc, prec=1)
+    finally:
+        hooks().ecgt_set_f32_kernel(prev)
+    K = np.prod([ext[c] for c in la if c not in lc])
+    bound = 4.0 * np.sqrt(K) * 2.0**-23 * np.einsum(f"{la},{lb}->{lc}", np.abs(A), np.abs(B)).max()
+    err = float(np.max(np.abs(got - ref)))
+    assert err < bound
+    assert err > 0.0
+
