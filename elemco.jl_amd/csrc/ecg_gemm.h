@@ -121,8 +121,11 @@ struct GemmEngine {
   void run_bf16x3(const GemmOp& op);
   // FP32 products on FP32 planes packed on the fly (ecg_gemm_bf16.hip, the (T)'s FP32 kernel)
   void run_f32p(const GemmOp& op);
-  void run_planes(const GemmOp& op, bool f32);
+  // FP64 products on FP64 planes packed on the fly (K slabs of 16)
+  void run_f64p(const GemmOp& op);
+  void run_planes(const GemmOp& op, int kind);   // kind 0 BF16 hi/lo, 1 FP32, 2 FP64 planes
   int f32_kernel = 1;   // large FP32 products: 0 generic FP32 MFMA kernel, 1 packed FP32 planes
+  int f64_kernel = 1;   // large FP64 products: 0 generic FP64 MFMA kernel, 1 packed FP64 planes
   // packed B planes kept for products with GemmOp::b_keep (one operand, identified by its
   // addressing); drop_kept_planes() forgets them (the caller's B may change afterwards)
   DevBuf pk_keep;

@@ -35,6 +35,8 @@ namespace {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
 constexpr int XM = 256, XN = 128, XK = 32;   // output tile and K slab
@@ -141,6 +143,45 @@ __global__ __launch_bounds__(256) void pack_f32p_kernel(PackArgs p, int nrt) {
   *reinterpret_cast<f4_t*>(dst + 4) = v1;
 }
 
+// FP64 plane for gemm_f64p_dma_kernel, K slabs of 16: position 4*(k%4) + k/4 of each 16-wide
+// slab holds k (so a lane's four k values of a slab, k = 4 s + lane/16, are two consecutive
+// 16-byte chunks); grid, reads and K segments as pack_bf16x3_kernel (a 32-wide tile = two
+// slabs).  PackArgs::f32 points at the plane's bytes.
+__global__ __launch_bounds__(256) void pack_f64p_kernel(PackArgs p, int nrt) {
+  __shared__ double fs[XK][PR + 1];
+  const int t = threadIdx.x;
+  const long bid = blockIdx.x;
+  const int r0 = (int)(bid % nrt) * PR, k0 = (int)(bid / nrt) * XK;
+  const double* src = p.src[0];
+  long sk = p.sk[0];
+  int kl0 = k0, klen = p.sK[0];
+#pragma unroll
+  for (int q = 1; q < MAXSEG; ++q)
+    if (q < p.nseg && k0 >= p.kp[q]) { src = p.src[q]; sk = p.sk[q]; kl0 = k0 - p.kp[q]; klen = p.sK[q]; }
+  const int row = t & (PR - 1);
+  const int r = r0 + row;
+  const bool rok = r < p.rows;
+  const long roff = rok ? split_off(r, p.d0, p.s0, p.s1) : 0;
+#pragma unroll
+  for (int j = 0; j < XK / 4; ++j) {
+    const int kk = (t >> 6) + 4 * j;
+    const int k = kl0 + kk;
+    const bool ok = rok && k < klen;
+    const double x = src[ok ? roff + (long)k * sk : 0];
+    fs[kk][row] = ok ? x : 0.0;
+  }
+  __syncthreads();
+  // write: row r0 + t/4, positions 8q .. 8q+7 of the tile: slab q/2, in-slab positions
+  // 8 (q%2) + e = 4 a + b with k%16 = a + 4 b, a = 2 (q%2) + e/4, b = e%4
+  const int wr = t >> 2, q = t & 3;
+  double v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = fs[16 * (q >> 1) + 2 * (q & 1) + (e >> 2) + 4 * (e & 3)][wr];
+  double* dst = reinterpret_cast<double*>(p.f32) + (size_t)(r0 + wr) * p.kpad + k0 + 8 * q;
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) *reinterpret_cast<d2_t*>(dst + e) = d2_t{v[e], v[e + 1]};
+}
+
 struct X3Params {
   int M, N, kpad, kchunk, ksplit, tiles_m, tiles_n;
   // K segments: slab k0 in [kb[s], kb[s+1]) reads row m of A at Ah[s] + m*lda[s] + (k0 - kb[s])
@@ -169,8 +210,10 @@ __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 
 // col + 16*(row/4), slot row%4 (TO: lane row + 16*(col/4), slot col%4).  Per
 // column group: all 16 offsets, then all loads of C, then all stores (a load
 // issued after a store to C could alias it, which would serialise every element).
-template <bool TO>
-__device__ __forceinline__ void x3_epilogue(const X3Params& p, const f4_t (&acc)[4][4], int bm, int bn, int wm,
+// L64: the FP64 16x16x4 MFMA's layout (slot r of lane (li, lk) holds row lk + 4 r of
+// column li) instead of the FP32/BF16 one (row 4 lk + r).
+template <bool TO, typename A4 = f4_t, bool L64 = false>
+__device__ __forceinline__ void x3_epilogue(const X3Params& p, const A4 (&acc)[4][4], int bm, int bn, int wm,
                                             int wn, int li, int lk, int kz, long zb) {
   double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0) + zb * p.cbz;
   float* Cf = p.Cf ? p.Cf + zb * p.cbz : nullptr;
@@ -182,8 +225,9 @@ __device__ __forceinline__ void x3_epilogue(const X3Params& p, const f4_t (&acc)
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = bn + wn * 64 + j * 16 + (TO ? 4 * lk + r : li);
-        const int m = bm + wm * 64 + i * 16 + (TO ? li : 4 * lk + r);
+        const int q = L64 ? lk + 4 * r : 4 * lk + r;
+        const int n = bn + wn * 64 + j * 16 + (TO ? q : li);
+        const int m = bm + wm * 64 + i * 16 + (TO ? li : q);
         const int e = 4 * i + r;
         ok[e] = n < p.N && m < p.M;
         long oe;
@@ -663,15 +707,119 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
   x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
 }
 
+constexpr int XK64 = 16;   // K slab of the FP64 packed product (128 bytes per row, as FP32's 32)
+
+// FP64 product on packed FP64 planes (pack_f64p_kernel; one plane per operand, single K
+// segment, plane pointers in X3Params::Ah / Bh, row strides kpad doubles): the tile, waves,
+// three LDS buffers, SGPR-base direct-to-LDS slab loads and swizzle of gemm_f32p_dma_kernel
+// with 16-wide K slabs of doubles and v_mfma_f64_16x16x4f64 (64 per slab and wave, as
+// many MFMA cycles as the FP32 kernel's 128).
+template <bool TO>
+__global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
+  __shared__ u32x4_t As[3][XM][8];   // [buffer][row][16-byte chunk]
+  __shared__ u32x4_t Bs[3][XN][8];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave index in an SGPR
+  const int wm = wave & 3, wn = wave >> 2;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
+  const int bm = tm * XM, bn = tn * XN;
+  const int kz = blockIdx.z;
+  const long zb = blockIdx.y;
+  const int kbeg = kz * p.kchunk;
+  const int kend = min(p.kpad, kbeg + p.kchunk);
+  const double* pa = reinterpret_cast<const double*>(p.Ah[0]);
+  const double* pb = reinterpret_cast<const double*>(p.Bh[0]);
+  const long ld = p.kpad;
+
+  // slab k0 -> one buffer: blocks of 8 rows x 128 bytes as in gemm_f32p_dma_kernel
+  const int lrow = lane >> 3, lch = (lane & 7) ^ lrow;
+  const unsigned off = (unsigned)(lrow * ld + 2 * lch) * 8u;
+  auto issue = [&](int buf, int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r0 = 8 * (wave + 8 * j);
+      dma16s(pa + (size_t)(bm + r0) * ld + k0, off, (lds_vptr)&As[buf][r0][0]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int r0 = 8 * (wave + 8 * j);
+      dma16s(pb + (size_t)(bn + r0) * ld + k0, off, (lds_vptr)&Bs[buf][r0][0]);
+    }
+  };
+
+  d4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+  const int li = lane & 15, lk = lane >> 4;
+  // a slab in two halves of two k steps (chunk 2 lk + h holds k steps 2 h and 2 h + 1)
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      d2_t a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = wm * 64 + i * 16 + li;
+        a[i] = __builtin_bit_cast(d2_t, As[buf][r][(2 * lk + h) ^ (r & 7)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = wn * 64 + j * 16 + li;
+        b[j] = __builtin_bit_cast(d2_t, Bs[buf][c][(2 * lk + h) ^ (c & 7)]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            if constexpr (TO) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[j][s], a[i][s], acc[i][j], 0, 0, 0);
+            else acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+          }
+    }
+  };
+  auto slab = [&](int k0) { return k0 < kend ? k0 : kbeg; };
+  auto wait_prev = []() { asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); };
+  auto barrier = []() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+  // one slab per iteration with the buffer index a plain variable (the loads are inline
+  // assembly the compiler does not track): the accumulators keep their registers across
+  // iterations, which the three-way unrolled loop of the FP32 kernel does not afford here
+  if (kbeg < kend) {
+    issue(0, kbeg);
+    issue(1, slab(kbeg + XK64));
+    wait_prev();
+  }
+  barrier();
+  int buf = 0;
+#pragma nounroll
+  for (int k0 = kbeg; k0 < kend; k0 += XK64) {
+    issue(buf == 0 ? 2 : buf - 1, slab(k0 + 2 * XK64));   // the buffer of slab k0 - XK64
+    compute(buf);
+    wait_prev();
+    barrier();
+    buf = buf == 2 ? 0 : buf + 1;
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
+  x3_epilogue<TO, d4_t, true>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
+}
+
 }  // namespace
 
-void GemmEngine::run_bf16x3(const GemmOp& op) { run_planes(op, false); }
+void GemmEngine::run_bf16x3(const GemmOp& op) { run_planes(op, 0); }
 
-void GemmEngine::run_f32p(const GemmOp& op) { run_planes(op, true); }
+void GemmEngine::run_f32p(const GemmOp& op) { run_planes(op, 1); }
 
-// Operands packed on the fly (BF16 hi/lo planes, or one FP32 plane for f32), then the
-// packed-plane kernel; split K when the tiles alone cannot fill the GPU.
-void GemmEngine::run_planes(const GemmOp& op, bool f32) {
+void GemmEngine::run_f64p(const GemmOp& op) { run_planes(op, 2); }
+
+// Operands packed on the fly (kind 0: BF16 hi/lo planes; 1: one FP32 plane; 2: one FP64
+// plane), then the packed-plane kernel; split K when the tiles alone cannot fill the GPU.
+void GemmEngine::run_planes(const GemmOp& op, int kind) {
   ECG_CHECK(op.batch == 1, "gemm: batched packed products are not supported");
   GemmSeg one;
   one.A = op.A; one.a_k = op.a_k; one.B = op.B; one.b_k = op.b_k; one.K = op.K;
@@ -699,11 +847,13 @@ void GemmEngine::run_planes(const GemmOp& op, bool f32) {
   bytes += 8.0 * ((double)(op.M + op.N) * ktot + (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N);
   const int tiles_m = (op.M + XM - 1) / XM, tiles_n = (op.N + XN - 1) / XN;
   const long Mp = (long)tiles_m * XM, Np = (long)tiles_n * XN;
-  u16 *Ah, *Al, *Bh, *Bl;
+  // bytes per element of a packed operand (two BF16 planes or one FP32 plane: 4; FP64: 8)
+  const size_t esz = kind == 2 ? 8 : 4;
+  char *a_base, *b_base;
   bool pack_b = true;
   if (op.b_keep) {
     // B planes kept between products: reuse them when B's addressing is the kept one
-    std::vector<long> key{f32 ? 1L : 0L, nseg, op.N, kpad, op.b_s0, op.b_s1, op.b_d0};
+    std::vector<long> key{kind, nseg, op.N, kpad, op.b_s0, op.b_s1, op.b_d0};
     for (int s = 0; s < nseg; ++s) {
       key.push_back((long)(__UINTPTR_TYPE__)sg[s].B);
       key.push_back(sg[s].b_k);
@@ -713,30 +863,32 @@ void GemmEngine::run_planes(const GemmOp& op, bool f32) {
       pack_b = false;
     } else {
       keep_key.clear();   // invalid while the planes are being replaced
-      pk_keep.ensure(sizeof(u16) * 2 * (size_t)Np * kpad);
+      pk_keep.ensure(esz * (size_t)Np * kpad);
       keep_key = key;
     }
-    pk.ensure(sizeof(u16) * 2 * (size_t)Mp * kpad);
-    Ah = static_cast<u16*>(pk.p);
-    Al = Ah + (size_t)Mp * kpad;
-    Bh = static_cast<u16*>(pk_keep.p);
-    Bl = Bh + (size_t)Np * kpad;
+    pk.ensure(esz * (size_t)Mp * kpad);
+    a_base = static_cast<char*>(pk.p);
+    b_base = static_cast<char*>(pk_keep.p);
   } else {
-    pk.ensure(sizeof(u16) * 2 * (size_t)(Mp + Np) * kpad);
-    Ah = static_cast<u16*>(pk.p);
-    Al = Ah + (size_t)Mp * kpad;
-    Bh = Al + (size_t)Mp * kpad;
-    Bl = Bh + (size_t)Np * kpad;
+    pk.ensure(esz * (size_t)(Mp + Np) * kpad);
+    a_base = static_cast<char*>(pk.p);
+    b_base = a_base + esz * (size_t)Mp * kpad;
   }
-  // an FP32 plane takes the bytes of the two BF16 planes (4 per element): its rows start at
-  // the hi plane's address with a row length of kpad floats
+  u16* Ah = reinterpret_cast<u16*>(a_base);
+  u16* Al = Ah + (size_t)Mp * kpad;   // BF16 lo plane (kind 0)
+  u16* Bh = reinterpret_cast<u16*>(b_base);
+  u16* Bl = Bh + (size_t)Np * kpad;
+  // an FP32 or FP64 plane starts at the hi plane's address, rows of kpad elements
   pa.rows = op.M; pa.kpad = kpad; pa.s0 = op.a_s0; pa.s1 = op.a_s1; pa.d0 = op.a_d0; pa.hi = Ah; pa.lo = Al;
   pb.rows = op.N; pb.kpad = kpad; pb.s0 = op.b_s0; pb.s1 = op.b_s1; pb.d0 = op.b_d0; pb.hi = Bh; pb.lo = Bl;
   pa.f32 = reinterpret_cast<float*>(Ah);
   pb.f32 = reinterpret_cast<float*>(Bh);
   const long nba = (Mp / PR) * (long)(kpad / XK), nbb = (Np / PR) * (long)(kpad / XK);
   ECG_CHECK(nba < (1L << 31) && nbb < (1L << 31), "gemm: operand too large for the packing grid");
-  if (f32) {
+  if (kind == 2) {
+    hipLaunchKernelGGL(pack_f64p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+    if (pack_b) hipLaunchKernelGGL(pack_f64p_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
+  } else if (kind == 1) {
     hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
     if (pack_b) hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
   } else {
@@ -777,7 +929,9 @@ void GemmEngine::run_planes(const GemmOp& op, bool f32) {
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
   const dim3 grid((unsigned)tiles, 1, (unsigned)ksplit);
-  if (f32)
+  if (kind == 2)
+    hipLaunchKernelGGL((gemm_f64p_dma_kernel<false>), grid, dim3(512), 0, stream, x);
+  else if (kind == 1)
     hipLaunchKernelGGL((gemm_f32p_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
   else if (bf16_kernel == 1)
     hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);

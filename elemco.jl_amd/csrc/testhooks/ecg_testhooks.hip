@@ -128,6 +128,15 @@ int ecgt_sqrtinvchol(const double* A, int n, double tol, int* rank, double* M) {
   }
 }
 
+// Selects the FP64 GEMM kernel for large products of the hooks' engine
+// (GemmEngine::f64_kernel: 0 generic, 1 packed FP64 planes); returns the previous choice.
+int ecgt_set_f64_kernel(int k) {
+  Env& e = env();
+  const int prev = e.ops.gemm.f64_kernel;
+  e.ops.gemm.f64_kernel = k;
+  return prev;
+}
+
 // Selects the FP32 GEMM kernel for large products of the hooks' engine
 // (GemmEngine::f32_kernel: 0 generic, 1 packed FP32 planes); returns the previous choice.
 int ecgt_set_f32_kernel(int k) {

@@ -141,3 +141,22 @@ def test_contract_fp32_kernels(kernel, case):
     assert err < bound
     assert err > 0.0
 
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("case", range(len(BF16_CASES)))
+def test_contract_fp64_kernels(kernel, case):
+    """Large FP64 products through both FP64 kernels (generic FP64 MFMA; operands packed into
+    FP64 planes for the direct-to-LDS kernel, K from one slab to many with split K) against
+    numpy: FP64 rounding only (1e-12 relative to |A||B|)."""
+    ext, la, lb, lc = BF16_CASES[case]
+    rng = np.random.default_rng(29)
+    A, B, C = _rand(la, ext, rng), _rand(lb, ext, rng), _rand(lc, ext, rng)
+    ref = np.einsum(f"{la},{lb}->{lc}", A, B) + C
+    prev = hooks().ecgt_set_f64_kernel(kernel)
+    try:
+        got = gpu_contract(1.0, A, la, B, lb, 1.0, C, lc, prec=0)
+    finally:
+        hooks().ecgt_set_f64_kernel(prev)
+    scale = np.einsum(f"{la},{lb}->{lc}", np.abs(A), np.abs(B)).max()
+    assert float(np.max(np.abs(got - ref))) < 1e-12 * scale
+
