@@ -601,11 +601,22 @@ __device__ __forceinline__ SlabSrcF slab_src_f32(const X3Params& p, int k0, long
   return r;
 }
 
+// Slot of chunk c of row r in a 128-byte LDS row of the FP32 / FP64 plane kernels:
+// c ^ plane_swz(r), plane_swz a permutation of r % 8 that swaps its two low bits.  A
+// fragment read (ds_read_b128, lane l: row r0 + l % 16, chunk 2 (l / 16) + h) is served in
+// passes of 16 lanes, {0..7, 16..23} and its shifts by 8 and 32, i.e. rows r..r+7 at chunks c
+// and c ^ 2, over 256 bytes of banks (two rows).  The four even rows of a pass get swizzles
+// {0, 1, 4, 5} and the odd rows {2, 3, 6, 7}: no two of one parity have XOR 0 or 2, so the
+// 16 lanes hit 16 distinct 16-byte bank groups (the plain r % 8 swizzle leaves rows r, r+2
+// of a pass in one group: a two-way conflict on every read).  Passes of eight consecutive
+// lanes, or of lanes {0..3, 16..19, 32..35, 48..51}, are conflict-free under it as well.
+__device__ __forceinline__ int plane_swz(int r) { return (r & 4) | ((r & 1) << 1) | ((r >> 1) & 1); }
+
 // FP32 product on packed FP32 planes (pack_f32p_kernel): the tile, waves, LDS buffers,
 // direct HBM -> LDS slab loads and epilogue of gemm_bf16x3_dma_kernel; a slab row is 32
-// floats = 8 chunks of 16 bytes, chunk c of row r stored at slot c ^ (r % 8) (the rows that
-// a 16-lane read touches then hit distinct bank groups).  Per slab and wave: 16 reads of
-// 16 bytes and 128 v_mfma_f32_16x16x4f32 (eight k steps of the 4x4 tiles).
+// floats = 8 chunks of 16 bytes, chunk c of row r stored at slot c ^ plane_swz(r).  Per slab
+// and wave: 16 reads of 16 bytes and 128 v_mfma_f32_16x16x4f32 (eight k steps of the 4x4
+// tiles).
 template <bool TO, bool MS>
 __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
   __shared__ u32x4_t A0[XM][8], A1[XM][8], A2[XM][8];   // [row][16-byte chunk]
@@ -622,10 +633,11 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
   const int kend = min(p.kpad, kbeg + p.kchunk);
 
   // slab k0 -> one buffer: A is 32 blocks of 8 rows x 128 bytes, B 16 blocks; wave w moves
-  // blocks w, w+8, ... (4 of A, 2 of B).  Lane l of a block loads row 8 c + l/8 into slot
-  // l%8, i.e. chunk (l%8) ^ (l/8) of that row: a uniform block base plus a per-lane byte
-  // offset that is the same for every block of an operand.
-  const int lrow = lane >> 3, lch = (lane & 7) ^ lrow;
+  // blocks w, w+8, ... (4 of A, 2 of B).  Lane l of a block loads row 8 q + l/8 into slot
+  // l%8, i.e. the chunk that the swizzle stores there: a uniform block base plus a per-lane
+  // byte offset that is the same for every block of a wave.
+  // row r of a block (r = 8 q + l/8, q = wave + 8 j) has r % 8 = l/8
+  const int lrow = lane >> 3, lch = (lane & 7) ^ plane_swz(lrow);
   auto issue = [&](u32x4_t (&As)[XM][8], u32x4_t (&Bs)[XN][8], int k0) {
     const SlabSrcF g = slab_src_f32<MS>(p, k0, zb);
     const unsigned offa = (unsigned)(lrow * g.lda + 4 * lch) * 4u;
@@ -648,6 +660,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
   const int li = lane & 15, lk = lane >> 4;
+  const int sw = plane_swz(li);   // = plane_swz(r) of every fragment row r (r = li mod 16)
   // a slab in two halves of four k steps (chunks 2 lk + h): 32 fragment registers live
   auto compute = [&](const u32x4_t (&As)[XM][8], const u32x4_t (&Bs)[XN][8]) {
 #pragma unroll
@@ -656,12 +669,12 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = wm * 64 + i * 16 + li;
-        a[i] = __builtin_bit_cast(f4_t, As[r][(2 * lk + h) ^ (r & 7)]);
+        a[i] = __builtin_bit_cast(f4_t, As[r][(2 * lk + h) ^ sw]);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int c = wn * 64 + j * 16 + li;
-        b[j] = __builtin_bit_cast(f4_t, Bs[c][(2 * lk + h) ^ (c & 7)]);
+        b[j] = __builtin_bit_cast(f4_t, Bs[c][(2 * lk + h) ^ sw]);
       }
 #pragma unroll
       for (int s = 0; s < 4; ++s)
@@ -733,7 +746,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
   const long ld = p.kpad;
 
   // slab k0 -> one buffer: blocks of 8 rows x 128 bytes as in gemm_f32p_dma_kernel
-  const int lrow = lane >> 3, lch = (lane & 7) ^ lrow;
+  const int lrow = lane >> 3, lch = (lane & 7) ^ plane_swz(lrow);
   const unsigned off = (unsigned)(lrow * ld + 2 * lch) * 8u;
   auto issue = [&](int buf, int k0) {
 #pragma unroll
@@ -754,6 +767,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
   const int li = lane & 15, lk = lane >> 4;
+  const int sw = plane_swz(li);   // = plane_swz(r) of every fragment row r (r = li mod 16)
   // a slab in two halves of two k steps (chunk 2 lk + h holds k steps 2 h and 2 h + 1)
   auto compute = [&](int buf) {
 #pragma unroll
@@ -762,12 +776,12 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = wm * 64 + i * 16 + li;
-        a[i] = __builtin_bit_cast(d2_t, As[buf][r][(2 * lk + h) ^ (r & 7)]);
+        a[i] = __builtin_bit_cast(d2_t, As[buf][r][(2 * lk + h) ^ sw]);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int c = wn * 64 + j * 16 + li;
-        b[j] = __builtin_bit_cast(d2_t, Bs[buf][c][(2 * lk + h) ^ (c & 7)]);
+        b[j] = __builtin_bit_cast(d2_t, Bs[buf][c][(2 * lk + h) ^ sw]);
       }
 #pragma unroll
       for (int s = 0; s < 2; ++s)
