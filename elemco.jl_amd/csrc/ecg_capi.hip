@@ -227,6 +227,28 @@ int ecg_fcidump_df(ecg_fcidump* f, double rtol, int* naux, double* BL, double* B
   });
 }
 
+int ecg_fcidump_freeze(ecg_fcidump* f, const int* orbs, int norbs, int* ncore, int* nfrozvirt) {
+  if (!f || norbs < 0 || (norbs > 0 && !orbs)) return -1;
+  return host_guarded([&] {
+    const ecg::FreezeCount c = ecg::freeze_orbs_in_dump(f->fd, std::vector<int>(orbs, orbs + norbs));
+    if (c.ncore + c.nvirt > 0) {
+      f->df_done = false;
+      f->df = ecg::DfFactors();
+    }
+    if (ncore) *ncore = c.ncore;
+    if (nfrozvirt) *nfrozvirt = c.nvirt;
+    return 0;
+  });
+}
+
+int ecg_fcidump_write(const ecg_fcidump* f, const char* path, double tol) {
+  if (!f || !path) return -1;
+  return host_guarded([&] {
+    ecg::write_fcidump(f->fd, path, tol < 0.0 ? 1e-12 : tol);
+    return 0;
+  });
+}
+
 int ecg_df_ao_to_mo(const double* C, int nao, int nmo, const double* AAP, const double* PQ, int nfit, double tol,
                     int device, int* naux, double* B) {
   return host_guarded([&] {
@@ -291,7 +313,13 @@ int ecg_create_from_fcidump(ecg_ctx** out, const char* path, double rtol, int de
   int st = ecg_fcidump_read(path, &f);
   if (st != 0) return st;
   std::unique_ptr<ecg_fcidump> hold(f);
-  st = host_guarded([&] {
+  return ecg_create_from_fcidump_handle(out, f, rtol, device, workspace_bytes);
+}
+
+int ecg_create_from_fcidump_handle(ecg_ctx** out, ecg_fcidump* f, double rtol, int device, size_t workspace_bytes) {
+  if (!out || !f) return -1;
+  *out = nullptr;
+  int st = host_guarded([&] {
     const long nelec = f->fd.head_int("NELEC", -1);
     ECG_CHECK(nelec >= 0, "NELEC option not found in fcidump");
     ECG_CHECK(nelec % 2 == 0 && f->fd.head_int("MS2", 0) == 0,

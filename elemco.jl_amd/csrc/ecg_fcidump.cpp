@@ -360,6 +360,132 @@ Fcidump read_fcidump(const std::string& path) {
   return fd;
 }
 
+FreezeCount freeze_orbs_in_dump(Fcidump& fd, const std::vector<int>& orbs) {
+  const long n = fd.norb;
+  const long nelec = fd.head_int("NELEC", -1);
+  if (nelec < 0) fail("NELEC option not found in fcidump");
+  if (nelec % 2 != 0 || fd.head_int("MS2", 0) != 0) fail("freeze_orbs_in_dump: the closed-shell path needs an even NELEC and MS2 = 0");
+  const long nocc = nelec / 2;
+  std::vector<char> frozen(n, 0);
+  for (int o : orbs)
+    if (o >= 1 && o <= n) frozen[o - 1] = 1;
+  std::vector<long> core, kept;
+  FreezeCount cnt;
+  for (long p = 0; p < n; ++p) {
+    if (!frozen[p]) {
+      kept.push_back(p);
+    } else if (p < nocc) {
+      core.push_back(p);
+      ++cnt.ncore;
+    } else {
+      ++cnt.nvirt;
+    }
+  }
+  if (cnt.ncore == 0 && cnt.nvirt == 0) return cnt;
+  auto eri = [&](long p, long q, long r, long s) { return fd.eri[p + n * (q + n * (r + n * s))]; };
+  // core_fock[p,q] = sum_c 2 (pq|cc) - (pc|cq)
+  std::vector<double> cf((size_t)n * n, 0.0);
+  for (long q = 0; q < n; ++q)
+    for (long p = 0; p < n; ++p) {
+      double v = 0.0;
+      for (long c : core) v += 2.0 * eri(p, q, c, c) - eri(p, c, c, q);
+      cf[p + n * q] = v;
+    }
+  for (long c : core) fd.int0 += 2.0 * fd.h[c + n * c] + cf[c + n * c];
+  const long m = (long)kept.size();
+  std::vector<double> h((size_t)m * m), e((size_t)m * m * m * m);
+  for (long b = 0; b < m; ++b)
+    for (long a = 0; a < m; ++a) h[a + m * b] = fd.h[kept[a] + n * kept[b]] + cf[kept[a] + n * kept[b]];
+  for (long d = 0; d < m; ++d)
+    for (long c = 0; c < m; ++c)
+      for (long b = 0; b < m; ++b)
+        for (long a = 0; a < m; ++a) e[a + m * (b + m * (c + m * d))] = eri(kept[a], kept[b], kept[c], kept[d]);
+  fd.h.swap(h);
+  fd.eri.swap(e);
+  fd.norb = (int)m;
+  fd.head["NORB"] = {std::to_string(m)};
+  fd.head_type["NORB"] = 'i';
+  fd.head["NELEC"] = {std::to_string(nelec - 2 * cnt.ncore)};
+  fd.head_type["NELEC"] = 'i';
+  fd.head["ORBSYM"] = std::vector<std::string>(m, "1");
+  fd.head_type["ORBSYM"] = 'i';
+  return cnt;
+}
+
+void write_fcidump(const Fcidump& fd, const std::string& path, double tol) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) fail("cannot open " + path + " for writing");
+  struct Closer {
+    FILE* f;
+    ~Closer() {
+      if (f) std::fclose(f);
+    }
+  } closer{f};
+  auto join = [&](const std::string& key) {
+    const auto& v = fd.head.at(key);
+    const bool quote = fd.head_type.count(key) && fd.head_type.at(key) == 's';
+    std::string s;
+    for (size_t i = 0; i < v.size(); ++i) s += (i ? "," : "") + (quote ? "\"" + v[i] + "\"" : v[i]);
+    return s;
+  };
+  static const char* keys[] = {"NORB", "NELEC", "MS2", "ISYM", "ORBSYM"};
+  std::fprintf(f, "&FCI\n");
+  for (const char* k : keys)
+    if (fd.has(k)) std::fprintf(f, " %s=%s,\n", k, join(k).c_str());
+  for (char group : {'i', 'f', 's'})
+    for (const auto& kv : fd.head) {
+      const std::string& k = kv.first;
+      const auto t = fd.head_type.find(k);
+      if ((t == fd.head_type.end() ? 's' : t->second) != group) continue;
+      if (std::find(std::begin(keys), std::end(keys), k) != std::end(keys)) continue;
+      if ((k == "IUHF" || k == "ST" || k == "III") && !kv.second.empty() && std::strtod(kv.second[0].c_str(), nullptr) == 0.0)
+        continue;
+      std::fprintf(f, " %s=%s,\n", k.c_str(), join(k).c_str());
+    }
+  std::fprintf(f, "/\n");
+  const long n = fd.norb;
+  auto line = [&](double v, long i, long j, long k, long l) {
+    std::fprintf(f, "%23.15e %3ld %3ld %3ld %3ld\n", v, i, j, k, l);
+  };
+  auto eri = [&](long p, long q, long r, long s) { return fd.eri[p + n * (q + n * (r + n * s))]; };
+  // 1-based loops of the reference; the value of line "p q r s" is (pq|rs)
+  if (fd.simtra()) {
+    for (long p = 1; p <= n; ++p)
+      for (long q = 1; q <= n; ++q) {
+        for (long r = 1; r < p; ++r)
+          for (long s = 1; s <= n; ++s) {
+            const double v = eri(p - 1, q - 1, r - 1, s - 1);
+            if (std::fabs(v) > tol) line(v, p, q, r, s);
+          }
+        for (long s = 1; s <= q; ++s) {
+          const double v = eri(p - 1, q - 1, p - 1, s - 1);
+          if (std::fabs(v) > tol) line(v, p, q, p, s);
+        }
+      }
+    for (long p = 1; p <= n; ++p)
+      for (long q = 1; q <= n; ++q) {
+        const double v = fd.h[(p - 1) + n * (q - 1)];
+        if (std::fabs(v) > tol) line(v, p, q, 0, 0);
+      }
+  } else {
+    for (long p = 1; p <= n; ++p)
+      for (long q = 1; q <= p; ++q)
+        for (long r = 1; r <= p; ++r)
+          for (long s = 1; s <= r; ++s) {
+            if (r * (r - 1) / 2 + s > p * (p - 1) / 2 + q) continue;
+            const double v = eri(p - 1, q - 1, r - 1, s - 1);
+            if (std::fabs(v) > tol) line(v, p, q, r, s);
+          }
+    for (long p = 1; p <= n; ++p)
+      for (long q = 1; q <= p; ++q) {
+        const double v = fd.h[(p - 1) + n * (q - 1)];
+        if (std::fabs(v) > tol) line(v, p, q, 0, 0);
+      }
+  }
+  line(fd.int0, 0, 0, 0, 0);
+  if (std::ferror(f)) fail("write error on " + path);
+}
+
 DfFactors df_factorise(const std::vector<double>& eri, int n, bool hermitian, double rtol) {
   const long N = (long)n * n;
   if ((long)eri.size() != N * N) fail("df_factorise: eri must hold norb^4 values");

@@ -33,6 +33,30 @@ struct Fcidump {
 // integral lines (dump.jl:573-604).  ".gz" files are read through zlib.
 Fcidump read_fcidump(const std::string& path);
 
+// freeze_orbs_in_dump (src/integrals/dumptools.jl:20-62), closed-shell branch: the
+// orbitals of `orbs` (1-based) that lie in the occupied space (the lowest NELEC/2) are
+// folded into the integrals as a frozen core, those in the virtual space are dropped.
+// With c over the frozen occupied orbitals and core_fock[p,q] = sum_c 2 (pq|cc) - (pc|cq)
+// (gen_fock over all occupied minus gen_fock over the remaining ones): int0 += 2 sum_c
+// h[c,c] + sum_c core_fock[c,c], h <- (h + core_fock)[kept, kept], (pq|rs) <- its kept
+// block; NORB, NELEC lose the frozen orbitals and ORBSYM becomes all 1 (modify_header!,
+// dump.jl:182-196).  As in the reference, numbers outside 1..NORB and repeats match no
+// orbital and are ignored.  Returns the numbers of frozen occupied and virtual orbitals.
+struct FreezeCount {
+  int ncore = 0, nvirt = 0;
+};
+FreezeCount freeze_orbs_in_dump(Fcidump& fd, const std::vector<int>& orbs);
+
+// write_fcidump (dump.jl:752-945) for IUHF = 0: "&FCI", the keys NORB, NELEC, MS2, ISYM,
+// ORBSYM first, then the other integer, float and string keys (strings in quotes; IUHF,
+// ST and III left out when 0), "/"; then "%23.15e %3i %3i %3i %3i" lines: the two-electron
+// integrals (pq|rs) with |value| > tol in the reference's loop order (write_integrals2_normal:
+// p >= q, p >= r >= s, pair (rs) <= pair (pq); write_integrals2_simtra for ST > 0: all (p,q)
+// and (r,s) with r < p, or r = p and s <= q), the one-electron integrals (q <= p, or all
+// (p,q) for ST > 0), and the core energy with indices 0 0 0 0.  The reference iterates each
+// typed group of the header in its Dict order; here each group is written in key order.
+void write_fcidump(const Fcidump& fd, const std::string& path, double tol);
+
 // Exact factorisation (pq|rs) = sum_Q BL[p,q,Q] BR[r,s,Q] of the (pq),(rs)
 // supermatrix (symmetric by particle symmetry) by diagonally pivoted LDL^T:
 // BL = L sqrt|D|, BR = L sign(D) sqrt|D|.  Pivots stop once the largest remaining

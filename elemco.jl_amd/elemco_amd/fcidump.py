@@ -44,9 +44,62 @@ class FDump:
     def simtra(self):
         return self._native.st > 0
 
+    def _sync(self):
+        n = self._native
+        self.head, self.int1, self.eri, self.int0 = n.head, n.int1, n.eri, n.int0
+
+    def freeze_orbs(self, orbs):
+        """``freeze_orbs_in_dump(EC, freeze_orbs)`` (``src/integrals/dumptools.jl:20-62``), in
+        place: 1-based orbital numbers; returns ``(ncore, nfrozen_virtual)``."""
+        out = self._native.freeze(orbs)
+        self._sync()
+        return out
+
+    def write(self, path: str, tol: float = 1e-12):
+        """``write_fcidump(fd, fcidump, tol)`` (``src/integrals/dump.jl:752-945``)."""
+        self._native.write(path, tol)
+
     def df_factors(self, rtol: float = 0.0):
         """Exact DF factors ``(BL, BR)``, n x n x naux; ``BR`` is None when it equals ``BL``."""
         return self._native.df_factors(rtol)
+
+
+def freeze_orbs_in_dump(fd: FDump, freeze_orbs) -> FDump:
+    """``freeze_orbs_in_dump(EC, freeze_orbs)`` (``src/integrals/dumptools.jl:20-62``): ``freeze_orbs``
+    is an iterable of 1-based orbital numbers or a string such as ``"1-4+6"`` / ``"1:4;6"``
+    (``parse_orbstring``)."""
+    if isinstance(freeze_orbs, str):
+        freeze_orbs = parse_orbstring(freeze_orbs)
+    fd.freeze_orbs(freeze_orbs)
+    return fd
+
+
+def parse_orbstring(s: str):
+    """Orbital list from ``"1-4+6-8"`` or ``"1:4;6:8"`` (both ``[1,2,3,4,6,7,8]``; ``"-3"`` is
+    ``1-3``): ``parse_orbstring`` (``src/infos/ecinfos.jl:606-660``) without orbital symmetry
+    labels (``"orb.sym"`` raises)."""
+    t = s.replace(" ", "").replace("-", ":").replace("+", ";")
+    if "." in t:
+        raise ValueError("orbital strings with symmetry labels are not supported: " + s)
+    if not t or any(c not in "0123456789:;" for c in t):
+        raise ValueError("Use only `0123456789:;+-` characters in the orbstring: " + s)
+    out = []
+    for part in t.split(";"):
+        if not part:
+            continue
+        if ":" in part:
+            a, b = part.split(":")
+            out += list(range(int(a) if a else 1, int(b) + 1))
+        else:
+            out.append(int(part))
+    if len(set(out)) != len(out):
+        raise ValueError("Repeated orbitals found in orbstring " + s)
+    return sorted(out)
+
+
+def write_fcidump(fd: FDump, fcidump: str, tol: float = 1e-12):
+    """``write_fcidump(fd, fcidump, tol)`` (``src/integrals/dump.jl:752-758``)."""
+    fd.write(fcidump, tol)
 
 
 def read_fcidump(path: str) -> FDump:

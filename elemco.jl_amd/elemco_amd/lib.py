@@ -67,6 +67,9 @@ _SIGS = {
     "ecg_fcidump_df": [_vp, ctypes.c_double, _ip, _dp, _dp, _ip],
     "ecg_create_from_fcidump": [ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_size_t],
+    "ecg_create_from_fcidump_handle": [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_int, ctypes.c_size_t],
+    "ecg_fcidump_freeze": [_vp, _ip, ctypes.c_int, _ip, _ip],
+    "ecg_fcidump_write": [_vp, ctypes.c_char_p, ctypes.c_double],
     "ecg_dims": [_vp, _ip, _ip, _ip],
     "ecg_df_ao_to_mo": [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
                         _dp],
@@ -148,6 +151,10 @@ class Fcidump:
         if lib.ecg_fcidump_read(os.fsencode(path), ctypes.byref(handle)) != 0:
             raise ElemcoError(f"ecg_fcidump_read failed: {lib.ecg_last_error(None).decode()}")
         self._h, self._lib = handle, lib
+        self._refresh()
+
+    def _refresh(self):
+        lib, handle = self._lib, self._h
         norb, nelec, ms2, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         e0 = ctypes.c_double()
         self._chk(lib.ecg_fcidump_info(handle, ctypes.byref(norb), ctypes.byref(nelec), ctypes.byref(ms2),
@@ -171,6 +178,22 @@ class Fcidump:
     def _chk(self, st):
         if st != 0:
             raise ElemcoError(self._lib.ecg_last_error(None).decode())
+
+    def freeze(self, orbs):
+        """``freeze_orbs_in_dump(EC, orbs)`` (``src/integrals/dumptools.jl:20-62``) on this dump
+        (``ecg_fcidump_freeze``): 1-based orbitals; occupied ones become a frozen core folded into
+        ``int1`` and ``int0``, virtual ones are dropped.  Returns ``(ncore, nfrozen_virtual)``."""
+        arr = np.ascontiguousarray(np.asarray(list(orbs), dtype=np.int32).reshape(-1))
+        ncore, nvirt = ctypes.c_int(), ctypes.c_int()
+        ptr = arr.ctypes.data_as(_ip) if arr.size else None
+        self._chk(self._lib.ecg_fcidump_freeze(self._h, ptr, int(arr.size), ctypes.byref(ncore), ctypes.byref(nvirt)))
+        self._refresh()
+        return ncore.value, nvirt.value
+
+    def write(self, path, tol=1e-12):
+        """``write_fcidump(fd, fcidump, tol)`` (``src/integrals/dump.jl:752-945``) via
+        ``ecg_fcidump_write``."""
+        self._chk(self._lib.ecg_fcidump_write(self._h, os.fsencode(path), float(tol)))
 
     def df_factors(self, rtol=0.0):
         """Exact DF factors ``(BL, BR)`` (``BR`` None when it equals ``BL``) of the integrals
@@ -220,8 +243,12 @@ class Context:
         factors and ``ecg_create`` with nocc = NELEC / 2 in one native call)."""
         lib = load()
         handle = _vp()
-        st = lib.ecg_create_from_fcidump(ctypes.byref(handle), os.fsencode(path), float(rtol), int(device),
-                                         int(workspace_bytes))
+        if isinstance(path, Fcidump):   # a dump in memory (e.g. after Fcidump.freeze)
+            st = lib.ecg_create_from_fcidump_handle(ctypes.byref(handle), path._h, float(rtol), int(device),
+                                                    int(workspace_bytes))
+        else:
+            st = lib.ecg_create_from_fcidump(ctypes.byref(handle), os.fsencode(path), float(rtol), int(device),
+                                             int(workspace_bytes))
         if st != 0:
             raise ElemcoError(f"ecg_create_from_fcidump failed ({st}): {lib.ecg_last_error(None).decode()}")
         self = cls.__new__(cls)

@@ -95,17 +95,32 @@ function ECGContext(nocc::Integer, h::Matrix{Float64}, BL::Array{Float64,3}; BR=
 end
 
 """
-    ECGContext(fcidump::AbstractString; rtol=0.0, device=0)
+    ECGContext(fcidump::AbstractString; rtol=0.0, device=0, freeze_orbs=Int[])
 
 `read_fcidump` (src/integrals/dump.jl:326-346; text or NPY2/NPY1 integrals, plain or
 gzip-compressed), exact DF factors and `ecg_create` with nocc = NELEC/2 in one native
 call (`ecg_create_from_fcidump`): the set-up `ccdriver(EC, method; fcidump)` does
-(src/cc/ccdriver.jl:37-76) for the closed-shell path.
+(src/cc/ccdriver.jl:37-76) for the closed-shell path.  A non-empty `freeze_orbs`
+(1-based) applies `@freeze_orbs` first (`freeze_orbs_in_dump`,
+src/integrals/dumptools.jl:20-62; `ecg_fcidump_freeze`).
 """
-function ECGContext(fcidump::AbstractString; rtol=0.0, device=0)
+function ECGContext(fcidump::AbstractString; rtol=0.0, device=0, freeze_orbs=Int[])
   ref = Ref{Ptr{Cvoid}}(C_NULL)
-  st = ccall((:ecg_create_from_fcidump, LIB), Cint, (Ref{Ptr{Cvoid}}, Cstring, Cdouble, Cint, Csize_t),
-             ref, fcidump, rtol, device, 0)
+  if isempty(freeze_orbs)
+    st = ccall((:ecg_create_from_fcidump, LIB), Cint, (Ref{Ptr{Cvoid}}, Cstring, Cdouble, Cint, Csize_t),
+               ref, fcidump, rtol, device, 0)
+  else
+    fd = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(nothing, ccall((:ecg_fcidump_read, LIB), Cint, (Cstring, Ref{Ptr{Cvoid}}), fcidump, fd))
+    orbs = Cint.(collect(freeze_orbs))
+    st = ccall((:ecg_fcidump_freeze, LIB), Cint, (Ptr{Cvoid}, Ptr{Cint}, Cint, Ptr{Cint}, Ptr{Cint}),
+               fd[], orbs, length(orbs), C_NULL, C_NULL)
+    if st == 0
+      st = ccall((:ecg_create_from_fcidump_handle, LIB), Cint,
+                 (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Cdouble, Cint, Csize_t), ref, fd[], rtol, device, 0)
+    end
+    ccall((:ecg_fcidump_free, LIB), Cint, (Ptr{Cvoid},), fd[])
+  end
   _check(nothing, st)
   o, v, q = Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0)
   ccall((:ecg_dims, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}, Ref{Cint}), ref[], o, v, q)

@@ -90,6 +90,23 @@ int ecg_fcidump_df(ecg_fcidump* fd, double rtol, int* naux, double* BL, double* 
 /* ecg_fcidump_read + ecg_fcidump_df + ecg_create with nocc = NELEC / 2: the
  * ccdriver(EC, method; fcidump) set-up (src/cc/ccdriver.jl:37-76) in one call. */
 int ecg_create_from_fcidump(ecg_ctx** ctx, const char* path, double rtol, int device, size_t workspace_bytes);
+/* The same from a dump already in memory (read, possibly frozen); fd stays the caller's. */
+int ecg_create_from_fcidump_handle(ecg_ctx** ctx, ecg_fcidump* fd, double rtol, int device, size_t workspace_bytes);
+/* freeze_orbs_in_dump (src/integrals/dumptools.jl:20-62; the @freeze_orbs macro,
+ * src/ElemCo.jl:703-725), closed-shell branch: orbitals orbs[0..norbs-1] (1-based) of
+ * the occupied space (the lowest NELEC/2) become a frozen core folded into h and the
+ * core energy (core Fock sum_c 2 (pq|cc) - (pc|cq); int0 += 2 sum_c h_cc + core Fock_cc),
+ * those of the virtual space are dropped; the integrals keep the other orbitals in order,
+ * NORB and NELEC shrink, ORBSYM becomes all 1.  Numbers outside 1..NORB and repeats are
+ * ignored, as in the reference.  *ncore / *nfrozvirt (may be NULL): orbitals frozen in
+ * each space.  Drops DF factors computed before. */
+int ecg_fcidump_freeze(ecg_fcidump* fd, const int* orbs, int norbs, int* ncore, int* nfrozvirt);
+/* write_fcidump (src/integrals/dump.jl:752-945) for IUHF = 0: header (NORB, NELEC, MS2,
+ * ISYM, ORBSYM first; IUHF/ST/III omitted when 0), then "%23.15e %3i %3i %3i %3i" lines
+ * of (pq|rs) in the reference's loop order (8-fold unique set, or the particle-symmetric
+ * set for ST > 0), h, and the core energy; values with |v| <= tol are skipped
+ * (tol < 0: the reference's default 1e-12). */
+int ecg_fcidump_write(const ecg_fcidump* fd, const char* path, double tol);
 /* DF preparation in front of the path: MO factors from AO three-index integrals, as
  * generate_AO_DF_integrals / generate_3idx_integrals (src/integrals/dftools.jl:28-125)
  * after the integral library has produced them.  C: nao x nmo MO coefficients,

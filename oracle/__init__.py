@@ -9,6 +9,7 @@ It restates, in numpy, the algorithm of the reference files (paths relative to
 the ElemCo.jl v0.14.0 source tree):
 
 * ``src/integrals/dump.jl``       FCIDUMP reader/writer            -> fcidump.py
+* ``src/integrals/dumptools.jl``  freeze_orbs_in_dump              -> fcidump.py
 * ``src/cc/cc.jl``                dressing, residual, energies,
                                   MP2, iteration loop               -> ccsd.py
 * ``src/cc/cctools.jl``           Fock/HF energy, updates, norms    -> ccsd.py

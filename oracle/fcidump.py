@@ -131,3 +131,103 @@ def read_fcidump(path: str) -> FDump:
     if int0 is None:
         raise ValueError("No core energy found in fcidump. Incomplete file?")
     return FDump(head, int1, int2, int0)
+
+
+def _gen_fock(fd: FDump, occ) -> np.ndarray:
+    """``gen_fock`` (``src/scf/fockfactory.jl:27-30``) over the occupied orbitals ``occ``
+    (0-based): h[p,q] + sum_i 2 <pi|qi> - <pi|iq>."""
+    occ = list(occ)
+    g = fd.int2
+    f = fd.int1.copy()
+    for i in occ:
+        f += 2.0 * g[:, i, :, i] - g[:, i, i, :]
+    return f
+
+
+def freeze_orbs_in_dump(fd: FDump, freeze_orbs) -> FDump:
+    """``freeze_orbs_in_dump`` closed-shell branch (``src/integrals/dumptools.jl:20-62``),
+    in place.  ``freeze_orbs``: 1-based orbital numbers; the occupied space is the lowest
+    NELEC/2 orbitals (``setup_space_fd!``, ``ecinfos.jl:171-192``, default occupations)."""
+    norb, nelec = fd.norb, fd.nelec
+    nocc = nelec // 2
+    occ = list(range(nocc))
+    full_fock = _gen_fock(fd, occ)
+    fr = {o - 1 for o in freeze_orbs}
+    freeze_occ = [i for i in occ if i in fr]                       # intersect(SP['o'], freeze_orbs)
+    freeze_virt = [a for a in range(nocc, norb) if a in fr]        # intersect(SP['v'], freeze_orbs)
+    nonfrozen = [p for p in range(norb) if p not in freeze_occ and p not in freeze_virt]
+    core_fock = full_fock - _gen_fock(fd, [i for i in occ if i not in freeze_occ])
+    fd.int0 += 2 * sum(fd.int1[c, c] for c in freeze_occ) + sum(core_fock[c, c] for c in freeze_occ)
+    ix = np.ix_(nonfrozen, nonfrozen)
+    fd.int1 = fd.int1[ix] + core_fock[ix]
+    fd.int2 = fd.int2[np.ix_(nonfrozen, nonfrozen, nonfrozen, nonfrozen)]
+    m = len(nonfrozen)
+    fd.head["NORB"] = [m]                                           # modify_header! (dump.jl:182-196)
+    fd.head["NELEC"] = [nelec - 2 * len(freeze_occ)]
+    fd.head["ORBSYM"] = [1] * m
+    return fd
+
+
+def write_fcidump(fd: FDump, path: str, tol: float = 1e-12) -> None:
+    """``write_fcidump`` (``src/integrals/dump.jl:752-945``) for IUHF=0: header, two-electron
+    lines in the reference's loop order (``write_integrals2_normal`` / ``_simtra``), one-electron
+    lines (``write_integrals1``), core energy.  The value written on line "p q r s" is
+    <pr|qs> = (pq|rs).  Header keys after NORB, NELEC, MS2, ISYM, ORBSYM go by type
+    (int, float, string) and name."""
+    keys = ["NORB", "NELEC", "MS2", "ISYM", "ORBSYM"]
+    out = ["&FCI"]
+
+    def fmt(v):
+        return f'"{v}"' if isinstance(v, str) else (repr(v) if isinstance(v, float) else str(v))
+
+    for k in keys:
+        if k in fd.head:
+            out.append(f" {k}={','.join(fmt(v) for v in fd.head[k])},")
+
+    def group(vals):
+        v = vals[0] if vals else ""
+        return 0 if isinstance(v, int) else (1 if isinstance(v, float) else 2)
+
+    rest = sorted((k for k in fd.head if k not in keys), key=lambda k: (group(fd.head[k]), k))
+    for k in rest:
+        vals = fd.head[k]
+        if k in ("IUHF", "ST", "III") and vals and vals[0] == 0:
+            continue
+        out.append(f" {k}={','.join(fmt(v) for v in vals)},")
+    out.append("/")
+    n = fd.norb
+    g = fd.int2
+
+    def line(v, p, q, r, s):
+        out.append("%23.15e %3i %3i %3i %3i" % (v, p, q, r, s))
+
+    if fd.simtra:
+        for p in range(1, n + 1):
+            for q in range(1, n + 1):
+                for r in range(1, p):
+                    for s in range(1, n + 1):
+                        v = g[p - 1, r - 1, q - 1, s - 1]
+                        if abs(v) > tol:
+                            line(v, p, q, r, s)
+                for s in range(1, q + 1):
+                    v = g[p - 1, p - 1, q - 1, s - 1]
+                    if abs(v) > tol:
+                        line(v, p, q, p, s)
+        pairs1 = [(p, q) for p in range(1, n + 1) for q in range(1, n + 1)]
+    else:
+        for p in range(1, n + 1):
+            for q in range(1, p + 1):
+                for r in range(1, p + 1):
+                    for s in range(1, r + 1):
+                        if r * (r - 1) // 2 + s <= p * (p - 1) // 2 + q:
+                            v = g[p - 1, r - 1, q - 1, s - 1]
+                            if abs(v) > tol:
+                                line(v, p, q, r, s)
+        pairs1 = [(p, q) for p in range(1, n + 1) for q in range(1, p + 1)]
+    for p, q in pairs1:
+        v = fd.int1[p - 1, q - 1]
+        if abs(v) > tol:
+            line(v, p, q, 0, 0)
+    line(fd.int0, 0, 0, 0, 0)
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")

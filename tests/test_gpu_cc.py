@@ -201,3 +201,33 @@ def test_graph_replayed_residual_matches_eager(method, prec):
     R1, R2 = EC.ctx.cc_residual(T10, T20, method)
     e3, _, _, it3, _ = EC.ctx.cc_solve(method, prec, E.default_opts(graphs=1, **opts))
     assert it3 == it0 and abs(e3[0] - e0[0]) < 1e-12
+
+
+@pytest.mark.parametrize("name,method,orbs", [("H2O.FCIDUMP.gz", E.ECG_CCSD, [1]),
+                                              ("H2O_ST1.FCIDUMP.gz", E.ECG_DCSD, [1]),
+                                              ("H2O.FCIDUMP.gz", E.ECG_DCSD, [1, 23])])
+def test_frozen_orbitals_match_oracle(name, method, orbs):
+    """@freeze_orbs (freeze_orbs_in_dump, dumptools.jl:20-62; test/h2o_st1.jl:33) before the
+    CC solve: the context made from the frozen dump in memory (ecg_fcidump_freeze,
+    ecg_create_from_fcidump_handle) gives the oracle's energy on the oracle-frozen dump, and
+    the HF energy of the full dump."""
+    from elemco_amd.lib import Context, Fcidump
+    from oracle.ccsd import ECInfo as OECInfo, ccdriver as occdriver
+    from oracle.fcidump import read_fcidump as oread, freeze_orbs_in_dump as ofreeze
+    path = os.path.join(GOLDEN, name)
+    full = Context.from_fcidump(path)
+    ehf_full = full.fock()[2]
+    full.close()
+    fd = Fcidump(path)
+    fd.freeze(orbs)
+    ctx = Context.from_fcidump(fd)
+    assert ctx.norb == fd.norb
+    assert abs(ctx.fock()[2] - ehf_full) < 1e-10
+    EC = E.ECInfo.from_fcidump(path)
+    opts = EC.opts()
+    e1 = float(ctx.cc_solve(method, E.ECG_FP64, opts)[0][0])
+    mname = {E.ECG_CCSD: "ccsd", E.ECG_DCSD: "dcsd"}[method]
+    eo = occdriver(OECInfo.from_fcidump(ofreeze(oread(path), orbs)), mname)
+    assert abs(e1 - eo[mname.upper() + "c"]) < 1e-9
+    ctx.close()
+    fd.close()

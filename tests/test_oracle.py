@@ -246,3 +246,52 @@ def test_dfprep_oracle_closed_form(rank):
     ref = np.einsum("mp,nq,mnP,PQ,lsQ,lr,st->pqrt", C, C, AAP, Pinv, AAP, C, C, optimize=True)
     assert np.max(np.abs(np.einsum("pqL,rsL->pqrs", B, B) - ref)) < 1e-9 * np.max(np.abs(ref))
 
+
+
+# ---------------------------------------------------------------- frozen orbitals, writer
+@pytest.mark.parametrize("name,orbs", [("H2O.FCIDUMP.gz", [1]), ("H2O.FCIDUMP.gz", [1, 2, 23]),
+                                       ("H2O_ST1.FCIDUMP.gz", [1]), ("H2O_ST1.FCIDUMP.gz", [2, 20])])
+def test_freeze_orbs_oracle_invariants(name, orbs):
+    """freeze_orbs_in_dump (dumptools.jl:20-62) against what freezing must leave unchanged:
+    the HF energy (asserted by the reference itself after @freeze_orbs, test/h2o_st1.jl:33-35;
+    against the reference's EHF for these files) and the Fock matrix (gen_fock over the
+    remaining occupied orbitals of the frozen dump = the full Fock on the kept orbitals)."""
+    from oracle.fcidump import _gen_fock, freeze_orbs_in_dump
+    ehf_ref = {"H2O.FCIDUMP.gz": -75.6457645933, "H2O_ST1.FCIDUMP.gz": -76.298014304953}[name]
+    full = read_fcidump(os.path.join(GOLDEN, name))
+    nocc = full.nelec // 2
+    f_full = _gen_fock(full, range(nocc))
+    kept = [p for p in range(full.norb) if p + 1 not in orbs]
+    fd = freeze_orbs_in_dump(read_fcidump(os.path.join(GOLDEN, name)), orbs)
+    ncore = sum(1 for o in orbs if o <= nocc)
+    assert fd.norb == full.norb - len(orbs) and fd.nelec == full.nelec - 2 * ncore
+    f_fr = _gen_fock(fd, range(nocc - ncore))
+    assert np.max(np.abs(f_fr - f_full[np.ix_(kept, kept)])) < 1e-12
+    ehf = lambda d, f, no: float(np.sum(np.diag(f)[:no]) + np.sum(np.diag(d.int1)[:no]) + d.int0)
+    e_full, e_fr = ehf(full, f_full, nocc), ehf(fd, f_fr, nocc - ncore)
+    assert abs(e_full - ehf_ref) < EPS_REF
+    assert abs(e_fr - e_full) < 1e-11
+    # a dropped core-Fock term in int0 breaks the energy invariant (planted in a copy)
+    bad = read_fcidump(os.path.join(GOLDEN, name))
+    freeze_orbs_in_dump(bad, orbs)
+    bad.int0 -= sum(f_full[c, c] - full.int1[c, c] for c in range(nocc) if c + 1 in orbs)
+    assert abs(ehf(bad, f_fr, nocc - ncore) - e_full) > 1e-3
+
+
+@pytest.mark.parametrize("name", ["H2O.FCIDUMP.gz", "H2O_ST1.FCIDUMP.gz"])
+def test_write_fcidump_oracle_roundtrip(tmp_path, name):
+    """write_fcidump (dump.jl:752-945) -> read_fcidump gives back every integral: the
+    written set (8-fold unique, or particle-symmetric for ST > 0) covers the array."""
+    from oracle.fcidump import write_fcidump as write_fd
+    fd = read_fcidump(os.path.join(GOLDEN, name))
+    for tol in (0.0, 1e-12):
+        out = str(tmp_path / f"out{tol}.FCIDUMP")
+        write_fd(fd, out, tol)
+        back = read_fcidump(out)
+        # IUHF, ST and III are not written when 0 (FDUMP_OPTIONAL, dump.jl:19)
+        opt0 = lambda h: {k: v for k, v in h.items() if not (k in ("IUHF", "ST", "III") and v and v[0] == 0)}
+        assert opt0(back.head) == opt0(fd.head)
+        keep2 = np.where(np.abs(fd.int2) > tol, fd.int2, 0.0)
+        keep1 = np.where(np.abs(fd.int1) > tol, fd.int1, 0.0)
+        assert np.max(np.abs(back.int2 - keep2)) < 1e-15 and np.max(np.abs(back.int1 - keep1)) < 1e-15
+        assert back.int0 == fd.int0
