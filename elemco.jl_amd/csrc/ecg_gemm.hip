@@ -299,10 +299,10 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     pr = Prec::F32;
   }
   if (pr == Prec::F64 && f64_kernel == 1 && op.batch == 1 &&
-      (double)op.M * op.N / ((double)op.M + op.N) >= 2048.0) {
+      (double)op.M * op.N / ((double)op.M + op.N) >= 1024.0) {
     // large FP64 products: operands packed into FP64 planes (16 bytes moved per element)
     // for the direct-to-LDS FP64 kernel.  The packing costs about 100 / (M N / (M + N)) of
-    // the product's time and the kernel gains about 7 %, hence the threshold.
+    // the product's time and the kernel gains about 14 %, hence the threshold.
     run_f64p(op);
     return;
   }
