@@ -401,7 +401,7 @@ def main():
                    "algorithmic_tflop_per_step": round(flops / 1e12, 3)},
         "roofline": {"bound": "mfma",
                      "kernel": "FP64 MFMA GEMM launches: ecg::gemm_kernel<double,...> and, for products with "
-                               "M N / (M + N) >= 2048, ecg::gemm_f64p_dma_kernel (its operand packing runs as "
+                               "M N / (M + N) >= 1024, ecg::gemm_f64p_dma_kernel (its operand packing runs as "
                                "separate HBM-bound pack_f64p_kernel launches, not counted here)",
                      "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["fp64"], 4), "traffic": measured_traffic(args.config),
