@@ -249,6 +249,22 @@ int ecg_fcidump_write(const ecg_fcidump* f, const char* path, double tol) {
   });
 }
 
+int ecg_df_freeze(int n, int nocc, int naux, const double* h, const double* BL, const double* BR, double e0,
+                  const int* orbs, int norbs, int* nkept, int* ncore, double* h_out, double* BL_out, double* BR_out,
+                  double* e0_out) {
+  return host_guarded([&] {
+    ECG_CHECK(norbs >= 0 && (norbs == 0 || orbs), "ecg_df_freeze: invalid orbital list");
+    const ecg::DfFreeze f = ecg::df_freeze(n, nocc, naux, h, BL, BR, e0, std::vector<int>(orbs, orbs + norbs));
+    if (nkept) *nkept = f.nkept;
+    if (ncore) *ncore = f.ncore;
+    if (e0_out) *e0_out = f.e0;
+    if (h_out) std::memcpy(h_out, f.h.data(), sizeof(double) * f.h.size());
+    if (BL_out) std::memcpy(BL_out, f.BL.data(), sizeof(double) * f.BL.size());
+    if (BR_out && BR) std::memcpy(BR_out, f.BR.data(), sizeof(double) * f.BR.size());
+    return 0;
+  });
+}
+
 int ecg_df_ao_to_mo(const double* C, int nao, int nmo, const double* AAP, const double* PQ, int nfit, double tol,
                     int device, int* naux, double* B) {
   return host_guarded([&] {

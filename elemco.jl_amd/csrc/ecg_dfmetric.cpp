@@ -1,4 +1,4 @@
-// sqrtinvchol of the DF metric (host C++; see ecg_dfprep.h).
+// sqrtinvchol of the DF metric and frozen orbitals on DF factors (host C++; see ecg_dfprep.h).
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -97,6 +97,64 @@ std::vector<double> sqrtinvchol(const double* A, int n, double tol, int& rank) {
   for (int a = 0; a < r; ++a)
     for (int i = 0; i < n; ++i) M[p[i] + (size_t)n * a] = x_(i, a);
   return M;
+}
+
+DfFreeze df_freeze(int n, int nocc, int naux, const double* h, const double* bl, const double* br, double e0,
+                   const std::vector<int>& orbs) {
+  if (n <= 0 || naux <= 0 || nocc < 0 || nocc > n) throw std::runtime_error("df_freeze: invalid sizes");
+  if (!h || !bl) throw std::runtime_error("df_freeze: NULL integrals");
+  const double* R = br ? br : bl;
+  std::vector<char> fr(n, 0);
+  for (int o : orbs)
+    if (o >= 1 && o <= n) fr[o - 1] = 1;
+  DfFreeze out;
+  std::vector<int> core, kept;
+  for (int p = 0; p < n; ++p) {
+    if (!fr[p]) kept.push_back(p);
+    else if (p < nocc) core.push_back(p);
+    else ++out.nvirt;
+  }
+  out.ncore = (int)core.size();
+  const size_t nn = (size_t)n * n;
+  auto L = [&](int p, int q, int Q) { return bl[p + (size_t)n * q + nn * Q]; };
+  auto Rr = [&](int p, int q, int Q) { return R[p + (size_t)n * q + nn * Q]; };
+  // d[Q] = sum_c R[c,c,Q]: the Coulomb part is sum_Q L[p,q,Q] d[Q]
+  std::vector<double> d(naux, 0.0);
+  for (int Q = 0; Q < naux; ++Q)
+    for (int c : core) d[Q] += Rr(c, c, Q);
+  std::vector<double> cf(nn, 0.0);
+  for (int Q = 0; Q < naux; ++Q) {
+    const double dq = 2.0 * d[Q];
+    const double* lq = bl + nn * Q;
+    for (size_t i = 0; i < nn; ++i) cf[i] += lq[i] * dq;
+  }
+  // exchange: cf[p,q] -= sum_{c,Q} L[p,c,Q] R[c,q,Q]
+  for (int Q = 0; Q < naux; ++Q)
+    for (int c : core)
+      for (int q = 0; q < n; ++q) {
+        const double r = Rr(c, q, Q);
+        if (r == 0.0) continue;
+        const double* lc = bl + (size_t)n * c + nn * Q;   // L[:, c, Q]
+        double* col = cf.data() + (size_t)n * q;
+        for (int p = 0; p < n; ++p) col[p] -= lc[p] * r;
+      }
+  out.e0 = e0;
+  for (int c : core) out.e0 += 2.0 * h[c + (size_t)n * c] + cf[c + (size_t)n * c];
+  const int m = (int)kept.size();
+  out.nkept = m;
+  out.h.resize((size_t)m * m);
+  for (int b = 0; b < m; ++b)
+    for (int a = 0; a < m; ++a)
+      out.h[a + (size_t)m * b] = h[kept[a] + (size_t)n * kept[b]] + cf[kept[a] + (size_t)n * kept[b]];
+  auto slice = [&](auto&& src, std::vector<double>& dst) {
+    dst.resize((size_t)m * m * naux);
+    for (int Q = 0; Q < naux; ++Q)
+      for (int b = 0; b < m; ++b)
+        for (int a = 0; a < m; ++a) dst[a + (size_t)m * b + (size_t)m * m * Q] = src(kept[a], kept[b], Q);
+  };
+  slice(L, out.BL);
+  if (br) slice(Rr, out.BR);
+  return out;
 }
 
 }  // namespace ecg

@@ -19,4 +19,20 @@ std::vector<double> sqrtinvchol(const double* A, int n, double tol, int& rank);
 void df_ao_to_mo_device(int device, const double* C, int nao, int nmo, const double* AAP, int nfit,
                         const double* M, int r, double* B);
 
+// Frozen orbitals on density-fitted integrals, the DF counterpart of freeze_orbs_in_dump
+// (src/integrals/dumptools.jl:20-62) and of freeze_core! / freeze_nvirt! after the Fock
+// matrix of all occupied orbitals (src/cc/ccdriver.jl:93-98, src/infos/ecinfos.jl:331-392):
+// (pq|rs) = sum_Q BL[p,q,Q] BR[r,s,Q] (BR = BL when br is null).  Orbitals of `orbs`
+// (1-based) below nocc form the core c: core_fock[p,q] = sum_c 2 (pq|cc) - (pc|cq),
+// e0 += 2 sum_c h[c,c] + core_fock[c,c], h <- (h + core_fock)[kept, kept], factors <- their
+// kept (p,q) block; frozen virtuals are dropped.  Numbers outside 1..n and repeats are
+// ignored.  Outputs: kept x kept (x naux) arrays, column-major.
+struct DfFreeze {
+  int ncore = 0, nvirt = 0, nkept = 0;
+  double e0 = 0.0;
+  std::vector<double> h, BL, BR;   // BR empty when br is null
+};
+DfFreeze df_freeze(int n, int nocc, int naux, const double* h, const double* bl, const double* br, double e0,
+                   const std::vector<int>& orbs);
+
 }  // namespace ecg

@@ -70,6 +70,8 @@ _SIGS = {
     "ecg_create_from_fcidump_handle": [ctypes.POINTER(_vp), _vp, ctypes.c_double, ctypes.c_int, ctypes.c_size_t],
     "ecg_fcidump_freeze": [_vp, _ip, ctypes.c_int, _ip, _ip],
     "ecg_fcidump_write": [_vp, ctypes.c_char_p, ctypes.c_double],
+    "ecg_df_freeze": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _ip, ctypes.c_int,
+                      _ip, _ip, _dp, _dp, _dp, _dp],
     "ecg_dims": [_vp, _ip, _ip, _ip],
     "ecg_df_ao_to_mo": [_dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip,
                         _dp],
@@ -139,6 +141,33 @@ def df_ao_to_mo(C, AAP, PQ, tol=1e-8, device=0):
     if st != 0:
         raise ElemcoError(f"ecg_df_ao_to_mo failed ({st}): {lib.ecg_last_error(None).decode()}")
     return np.asfortranarray(B[:, :, :naux.value])
+
+
+def df_freeze(h, BL, BR, e0, nocc, orbs):
+    """Frozen orbitals on DF integrals (``ecg_df_freeze``; ``freeze_core!`` / ``freeze_nvirt!`` after
+    the full Fock matrix, ``src/cc/ccdriver.jl:93-98``, as ``freeze_orbs_in_dump`` does on four-index
+    integrals, ``src/integrals/dumptools.jl:20-62``).  ``orbs``: 1-based.  Returns
+    ``(h, BL, BR, e0, nocc, nvirt)`` of the frozen system (``BR`` None when it was None)."""
+    lib = load()
+    n, _, naux = BL.shape
+    h = fortran(h, (n, n))
+    BL = fortran(BL, (n, n, naux))
+    BR = None if BR is None else fortran(BR, (n, n, naux))
+    arr = np.ascontiguousarray(np.asarray(list(orbs), dtype=np.int32).reshape(-1))
+    optr = arr.ctypes.data_as(_ip) if arr.size else None
+    nk, nc, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    args = (int(n), int(nocc), int(naux), _ptr(h), _ptr(BL), _ptr(BR) if BR is not None else None, float(e0), optr,
+            int(arr.size), ctypes.byref(nk), ctypes.byref(nc))
+    if lib.ecg_df_freeze(*args, None, None, None, None) != 0:
+        raise ElemcoError(f"ecg_df_freeze failed: {lib.ecg_last_error(None).decode()}")
+    m = nk.value
+    ho = np.zeros((m, m), order="F")
+    BLo = np.zeros((m, m, naux), order="F")
+    BRo = None if BR is None else np.zeros((m, m, naux), order="F")
+    if lib.ecg_df_freeze(*args, _ptr(ho), _ptr(BLo), _ptr(BRo) if BRo is not None else None, ctypes.byref(e)) != 0:
+        raise ElemcoError(f"ecg_df_freeze failed: {lib.ecg_last_error(None).decode()}")
+    no = int(nocc) - nc.value
+    return ho, BLo, BRo, e.value, no, m - no
 
 
 class Fcidump:

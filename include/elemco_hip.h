@@ -107,6 +107,18 @@ int ecg_fcidump_freeze(ecg_fcidump* fd, const int* orbs, int norbs, int* ncore, 
  * set for ST > 0), h, and the core energy; values with |v| <= tol are skipped
  * (tol < 0: the reference's default 1e-12). */
 int ecg_fcidump_write(const ecg_fcidump* fd, const char* path, double tol);
+/* Frozen orbitals on density-fitted integrals (host side): freeze_core! / freeze_nvirt!
+ * after the Fock matrix of all occupied orbitals (src/cc/ccdriver.jl:93-98,
+ * src/infos/ecinfos.jl:331-392), i.e. freeze_orbs_in_dump (src/integrals/dumptools.jl:20-62)
+ * on (pq|rs) = sum_Q BL[p,q,Q] BR[r,s,Q] (BR NULL: = BL).  orbs[0..norbs-1] (1-based) below
+ * nocc are the core, folded into h and the core energy (core Fock sum_c 2 (pq|cc) - (pc|cq)),
+ * the others are dropped; out of range numbers and repeats are ignored.  Outputs: *nkept,
+ * *ncore, h_out (nkept^2), BL_out / BR_out (nkept^2 * naux; BR_out only with BR), *e0_out;
+ * a call with NULL arrays returns the counts only.  The frozen system is then
+ * ecg_create(nocc - ncore, nkept - nocc + ncore, naux, h_out, BL_out, BR_out, e0_out, ...). */
+int ecg_df_freeze(int n, int nocc, int naux, const double* h, const double* BL, const double* BR, double e0,
+                  const int* orbs, int norbs, int* nkept, int* ncore, double* h_out, double* BL_out, double* BR_out,
+                  double* e0_out);
 /* DF preparation in front of the path: MO factors from AO three-index integrals, as
  * generate_AO_DF_integrals / generate_3idx_integrals (src/integrals/dftools.jl:28-125)
  * after the integral library has produced them.  C: nao x nmo MO coefficients,

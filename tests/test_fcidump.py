@@ -171,3 +171,24 @@ def test_parse_orbstring():
     for bad in ("1-3+2", "1,2", "1.1"):
         with pytest.raises(ValueError):
             parse_orbstring(bad)
+
+
+@pytest.mark.parametrize("name,orbs", [("H2O.FCIDUMP.gz", [1]), ("H2O.FCIDUMP.gz", [1, 2, 23]),
+                                       ("H2O_ST1.FCIDUMP.gz", [1]), ("H2O_ST1.FCIDUMP.gz", [2, 20])])
+def test_df_freeze_matches_four_index_freeze(name, orbs):
+    """ecg_df_freeze on the exact DF factors of a dump = freeze_orbs_in_dump on its four-index
+    integrals (the oracle's, itself pinned by the HF-energy and Fock invariants): h, core
+    energy, and the integrals rebuilt from the frozen factors."""
+    from oracle.fcidump import read_fcidump as oread, freeze_orbs_in_dump as ofreeze
+    path = os.path.join(GOLDEN, name)
+    fd = read_fcidump(path)
+    BL, BR = fd.df_factors(1e-14)
+    nocc = fd.nelec // 2
+    h, bl, br, e0, no, nv = lib.df_freeze(fd.int1, BL, BR, fd.int0, nocc, orbs)
+    od = ofreeze(oread(path), orbs)
+    assert (no + nv, 2 * no) == (od.norb, od.nelec)
+    assert abs(e0 - od.int0) < 1e-10
+    assert np.max(np.abs(h - od.int1)) < 1e-10
+    eri = np.einsum("pqQ,rsQ->pqrs", bl, bl if br is None else br)
+    assert np.max(np.abs(eri - od.int2.transpose(0, 2, 1, 3))) < 1e-10
+    assert (br is None) == (BR is None)

@@ -231,3 +231,27 @@ def test_frozen_orbitals_match_oracle(name, method, orbs):
     assert abs(e1 - eo[mname.upper() + "c"]) < 1e-9
     ctx.close()
     fd.close()
+
+
+@pytest.mark.parametrize("name,method,orbs", [("H2O.FCIDUMP.gz", E.ECG_CCSD, [1]),
+                                              ("H2O_ST1.FCIDUMP.gz", E.ECG_DCSD, [1, 20])])
+def test_df_freeze_energy_matches_frozen_dump(name, method, orbs):
+    """Frozen core on the DF route (ecg_df_freeze, then ecg_create) gives the energy of the
+    frozen four-index dump (ecg_fcidump_freeze, ecg_create_from_fcidump_handle)."""
+    from elemco_amd.lib import Context, Fcidump, df_freeze
+    path = os.path.join(GOLDEN, name)
+    fd = Fcidump(path)
+    BL, BR = fd.df_factors(1e-14)
+    h, bl, br, e0, no, nv = df_freeze(fd.int1, BL, BR, fd.int0, fd.nelec // 2, orbs)
+    c1 = Context(no, nv, bl.shape[2], h, bl, br, e0)
+    fd.freeze(orbs)
+    c2 = Context.from_fcidump(fd)
+    EC = E.ECInfo.from_fcidump(path)
+    opts = EC.opts()
+    e1 = float(c1.cc_solve(method, E.ECG_FP64, opts)[0][0])
+    e2 = float(c2.cc_solve(method, E.ECG_FP64, opts)[0][0])
+    assert abs(e1 - e2) < 1e-9
+    assert abs(c1.fock()[2] - c2.fock()[2]) < 1e-9
+    c1.close()
+    c2.close()
+    fd.close()
