@@ -24,7 +24,7 @@ one-electron integrals, or straight from an FCIDUMP file (`ECGContext(path)`).
 """
 module ElemCoAMD
 
-export ECGContext, calc_cc_resid, calc_ccsd_vector_times_Jacobian, calc_1RDM, calc_pertT, calc_cc, calc_lm_cc,
+export ECGContext, write_fcidump, df_freeze, calc_cc_resid, calc_ccsd_vector_times_Jacobian, calc_1RDM, calc_pertT, calc_cc, calc_lm_cc,
        energy_grad, SolverOpts, default_opts, ECG_FP64, ECG_FP32, ECG_BF16X3
 
 const LIB = get(ENV, "ELEMCO_HIP_LIB", joinpath(@__DIR__, "..", "libelemco_hip.so"))
@@ -259,6 +259,55 @@ function energy_grad(ctx::ECGContext, T1, T2, U1, U2, method::AbstractString)
              ctx.handle, m, T1, T2, U1, U2, gh, gb)
   _check(ctx, st)
   return gh, gb
+end
+
+"""
+    write_fcidump(fcidump_in, fcidump_out; freeze_orbs=Int[], tol=1e-12)
+
+`@freeze_orbs` (freeze_orbs_in_dump, src/integrals/dumptools.jl:20-62) then `@write_ints`
+(write_fcidump, src/integrals/dump.jl:752-945) on a dump read by the library's reader
+(`ecg_fcidump_read`, `ecg_fcidump_freeze`, `ecg_fcidump_write`).
+"""
+function write_fcidump(fcidump_in::AbstractString, fcidump_out::AbstractString; freeze_orbs=Int[], tol=1e-12)
+  fd = Ref{Ptr{Cvoid}}(C_NULL)
+  _check(nothing, ccall((:ecg_fcidump_read, LIB), Cint, (Cstring, Ref{Ptr{Cvoid}}), fcidump_in, fd))
+  orbs = Cint.(collect(freeze_orbs))
+  st = isempty(orbs) ? 0 : ccall((:ecg_fcidump_freeze, LIB), Cint, (Ptr{Cvoid}, Ptr{Cint}, Cint, Ptr{Cint}, Ptr{Cint}),
+                                 fd[], orbs, length(orbs), C_NULL, C_NULL)
+  if st == 0
+    st = ccall((:ecg_fcidump_write, LIB), Cint, (Ptr{Cvoid}, Cstring, Cdouble), fd[], fcidump_out, tol)
+  end
+  ccall((:ecg_fcidump_free, LIB), Cint, (Ptr{Cvoid},), fd[])
+  _check(nothing, st)
+  return nothing
+end
+
+"""
+    df_freeze(h, BL, BR, e0, nocc, freeze_orbs) -> (h, BL, BR, e0, nocc, nvirt)
+
+Frozen orbitals on DF integrals (`ecg_df_freeze`): `freeze_core!` / `freeze_nvirt!` after the
+Fock matrix of all occupied orbitals (src/cc/ccdriver.jl:93-98).  `BR === nothing` for
+symmetric factors.  The result feeds `ECGContext(nocc, h, BL; BR=BR, e0=e0)`.
+"""
+function df_freeze(h::Matrix{Float64}, BL::Array{Float64,3}, BR, e0::Real, nocc::Integer, freeze_orbs)
+  n, _, naux = size(BL)
+  orbs = Cint.(collect(freeze_orbs))
+  nk, nc, e = Ref{Cint}(0), Ref{Cint}(0), Ref{Cdouble}(0.0)
+  brp = BR === nothing ? Ptr{Cdouble}(C_NULL) : pointer(BR)
+  st = GC.@preserve h BL BR orbs ccall((:ecg_df_freeze, LIB), Cint, (Cint, Cint, Cint, Ptr{Cdouble}, Ptr{Cdouble},
+           Ptr{Cdouble}, Cdouble, Ptr{Cint}, Cint, Ref{Cint}, Ref{Cint}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+           Ref{Cdouble}), n, nocc, naux, h, BL, brp, e0, orbs, length(orbs), nk, nc,
+           C_NULL, C_NULL, C_NULL, e)
+  _check(nothing, st)
+  m = nk[]
+  ho = zeros(m, m); blo = zeros(m, m, naux); bro = BR === nothing ? nothing : zeros(m, m, naux)
+  st = GC.@preserve h BL BR orbs ho blo bro ccall((:ecg_df_freeze, LIB), Cint, (Cint, Cint, Cint, Ptr{Cdouble},
+           Ptr{Cdouble}, Ptr{Cdouble}, Cdouble, Ptr{Cint}, Cint, Ref{Cint}, Ref{Cint}, Ptr{Cdouble}, Ptr{Cdouble},
+           Ptr{Cdouble}, Ref{Cdouble}), n, nocc, naux, h, BL, brp, e0, orbs, length(orbs), nk, nc,
+           ho, blo, bro === nothing ? Ptr{Cdouble}(C_NULL) : pointer(bro), e)
+  _check(nothing, st)
+  no = nocc - nc[]
+  return ho, blo, bro, e[], no, m - no
 end
 
 end # module
