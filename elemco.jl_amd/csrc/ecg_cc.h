@@ -96,7 +96,7 @@ struct Context {
   double last_residual_ms = 0.0, last_ladder_ms = 0.0;
   int fp64_iterations = 0;   // FP64 iterations of the last solve (mixed precision)
   DevBuf tws[25];            // (T) work buffers: kept between calls, grown on demand
-  int triples_batch[2][2] = {{0, 0}, {0, 0}};   // measured (T) block size of the packed paths [FP32?][Lambda?]; 0: not yet
+  int triples_batch[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // measured (T) block size of the packed paths [BF16x3, FP32, FP64][Lambda?]; 0: not yet
   int triples_last_batch = 1;      // block size used by the last (T) call
   // Residual launch sequences captured as HIP graphs (SolverOpts::graphs), one per
   // (method, precision, buffers); a graph is recaptured when a scratch buffer it

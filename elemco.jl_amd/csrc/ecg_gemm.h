@@ -92,6 +92,9 @@ struct PackedOp {
   // f32: the planes are FP32 (pack_rows_f32; Ah / Bh hold the FP32 plane pointers, Al / Bl
   // are unused, row strides and batch offsets count floats) and the products are FP32 MFMA
   bool f32 = false;
+  // f64: FP64 planes (pack_rows_f64; plane pointers in Ah / Bh) multiplied by the FP64 MFMA
+  // kernel into the FP64 output C
+  bool f64 = false;
 };
 
 // Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,
@@ -140,6 +143,7 @@ struct GemmEngine {
   // Pack rows as pack_rows_bf16x3 into one FP32 plane [r][kpad]; within each 32-wide K slab
   // position 8*(k%4) + k/4 holds k (the order the FP32 packed product reads).
   void pack_rows_f32(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad, float* out);
+  void pack_rows_f64(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad, double* out);
   void reduce_slabs(const GemmOp& op, int ksplit);   // split-K partial slabs in ws -> C
   hipEvent_t timing_begin();
   void timing_end(hipEvent_t e0);

@@ -578,27 +578,35 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
 
 // FP32 planes and row strides (in floats) of the segment holding slab k0: the packed
 // FP32 product keeps its planes in X3Params::Ah / Bh (as FP32 pointers).
-struct SlabSrcF {
-  const float *a, *b;
+template <typename E>
+struct SlabSrcT {
+  const E *a, *b;
   long lda, ldb;
   int kl;
 };
-template <bool MS>
-__device__ __forceinline__ SlabSrcF slab_src_f32(const X3Params& p, int k0, long zb) {
-  SlabSrcF r{reinterpret_cast<const float*>(p.Ah[0]), reinterpret_cast<const float*>(p.Bh[0]), p.kpad, p.kpad, k0};
+using SlabSrcF = SlabSrcT<float>;
+// slab k0 of a packed FP32 / FP64 product: plane pointers, row strides and in-segment k of
+// the K segment holding k0 (one segment unless MS), advanced to batch member zb
+template <bool MS, typename E = float>
+__device__ __forceinline__ SlabSrcT<E> slab_src_planes(const X3Params& p, int k0, long zb) {
+  SlabSrcT<E> r{reinterpret_cast<const E*>(p.Ah[0]), reinterpret_cast<const E*>(p.Bh[0]), p.kpad, p.kpad, k0};
   if constexpr (MS) {
     r.lda = p.lda[0]; r.ldb = p.ldb[0];
     long oa = p.az[0], ob = p.bz[0];
 #pragma unroll
     for (int q = 1; q < MAXSEG; ++q)
       if (q < p.nseg && k0 >= p.kb[q]) {
-        r.a = reinterpret_cast<const float*>(p.Ah[q]); r.b = reinterpret_cast<const float*>(p.Bh[q]);
+        r.a = reinterpret_cast<const E*>(p.Ah[q]); r.b = reinterpret_cast<const E*>(p.Bh[q]);
         r.lda = p.lda[q]; r.ldb = p.ldb[q]; r.kl = k0 - p.kb[q];
         oa = p.az[q]; ob = p.bz[q];
       }
     r.a += zb * oa; r.b += zb * ob;
   }
   return r;
+}
+template <bool MS>
+__device__ __forceinline__ SlabSrcF slab_src_f32(const X3Params& p, int k0, long zb) {
+  return slab_src_planes<MS, float>(p, k0, zb);
 }
 
 // Slot of chunk c of row r in a 128-byte LDS row of the FP32 / FP64 plane kernels:
@@ -727,7 +735,7 @@ constexpr int XK64 = 16;   // K slab of the FP64 packed product (128 bytes per r
 // three LDS buffers, SGPR-base direct-to-LDS slab loads and swizzle of gemm_f32p_dma_kernel
 // with 16-wide K slabs of doubles and v_mfma_f64_16x16x4f64 (64 per slab and wave, as
 // many MFMA cycles as the FP32 kernel's 128).
-template <bool TO>
+template <bool TO, bool MS = false>
 __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
   __shared__ u32x4_t As[3][XM][8];   // [buffer][row][16-byte chunk]
   __shared__ u32x4_t Bs[3][XN][8];
@@ -749,15 +757,32 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
   const int lrow = lane >> 3, lch = (lane & 7) ^ plane_swz(lrow);
   const unsigned off = (unsigned)(lrow * ld + 2 * lch) * 8u;
   auto issue = [&](int buf, int k0) {
+    if constexpr (MS) {
+      // K segments and batch members (PackedOp with f64): per-slab plane, stride and offset
+      const SlabSrcT<double> g = slab_src_planes<true, double>(p, k0, zb);
+      const unsigned offa = (unsigned)(lrow * g.lda + 2 * lch) * 8u;
+      const unsigned offb = (unsigned)(lrow * g.ldb + 2 * lch) * 8u;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r0 = 8 * (wave + 8 * j);
-      dma16s(pa + (size_t)(bm + r0) * ld + k0, off, (lds_vptr)&As[buf][r0][0]);
-    }
+      for (int j = 0; j < 4; ++j) {
+        const int r0 = 8 * (wave + 8 * j);
+        dma16s(g.a + (size_t)(bm + r0) * g.lda + g.kl, offa, (lds_vptr)&As[buf][r0][0]);
+      }
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int r0 = 8 * (wave + 8 * j);
-      dma16s(pb + (size_t)(bn + r0) * ld + k0, off, (lds_vptr)&Bs[buf][r0][0]);
+      for (int j = 0; j < 2; ++j) {
+        const int r0 = 8 * (wave + 8 * j);
+        dma16s(g.b + (size_t)(bn + r0) * g.ldb + g.kl, offb, (lds_vptr)&Bs[buf][r0][0]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r0 = 8 * (wave + 8 * j);
+        dma16s(pa + (size_t)(bm + r0) * ld + k0, off, (lds_vptr)&As[buf][r0][0]);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int r0 = 8 * (wave + 8 * j);
+        dma16s(pb + (size_t)(bn + r0) * ld + k0, off, (lds_vptr)&Bs[buf][r0][0]);
+      }
     }
   };
 
@@ -986,6 +1011,21 @@ void GemmEngine::pack_rows_f32(const double* src, long rows, int d0, long s0, lo
   ECG_HIP(hipGetLastError());
 }
 
+void GemmEngine::pack_rows_f64(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad,
+                               double* out) {
+  ECG_CHECK(K > 0 && kpad % XK == 0 && kpad >= K, "pack: kpad must be a multiple of 32 covering K");
+  const long rt = (rows + PR - 1) / PR;
+  ECG_CHECK(rt * PR < (1L << 31) && rt * (kpad / XK) < (1L << 31), "pack: operand too large for the packing grid");
+  PackArgs pa{};
+  pa.rows = (int)rows; pa.kpad = kpad; pa.nseg = 1;
+  for (int s = 0; s < MAXSEG; ++s) { pa.src[s] = src; pa.sk[s] = sk; pa.sK[s] = K; }
+  pa.kp[0] = 0;
+  for (int s = 1; s <= MAXSEG; ++s) pa.kp[s] = kpad;
+  pa.s0 = s0; pa.s1 = s1; pa.d0 = d0; pa.f32 = reinterpret_cast<float*>(out);
+  hipLaunchKernelGGL(pack_f64p_kernel, dim3((unsigned)(rt * (kpad / XK))), dim3(256), 0, stream, pa, (int)rt);
+  ECG_HIP(hipGetLastError());
+}
+
 void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   ECG_CHECK(op.nseg >= 1 && op.nseg <= MAXSEG, "gemm: packed product needs 1..4 K segments");
   ECG_CHECK((op.C != nullptr) != (op.Cf != nullptr), "gemm: packed product needs exactly one output");
@@ -996,7 +1036,7 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   int kpad = 0;
   for (int s = 0; s < op.nseg; ++s) {
     const PackedSeg& g = op.seg[s];
-    ECG_CHECK(g.K > 0 && g.K % XK == 0 && g.Ah && g.Bh && (op.f32 || (g.Al && g.Bl)) && g.lda >= g.K &&
+    ECG_CHECK(g.K > 0 && g.K % XK == 0 && g.Ah && g.Bh && (op.f32 || op.f64 || (g.Al && g.Bl)) && g.lda >= g.K &&
                   g.ldb >= g.K,
               "gemm: packed segment needs K a positive multiple of 32 and row strides >= K");
     x.kb[s] = kpad;
@@ -1024,14 +1064,18 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
             "gemm: a boxed output needs M = box_v^2 and N = box_v");
   flops += 2LL * op.M * op.N * op.algorithmic_k * op.batch;
   bytes += op.batch * ((op.Cf ? 4.0 : 8.0) * (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N +
-                       4.0 * (double)(op.M + op.N) * op.algorithmic_k);
+                       (op.f64 ? 8.0 : 4.0) * (double)(op.M + op.N) * op.algorithmic_k);
   const long tiles = (long)x.tiles_m * x.tiles_n;
   ECG_CHECK(tiles < (1L << 31), "gemm: too many tiles");
   hipEvent_t e0 = timing_begin();
   // transposed accumulators when the stores run along m: unit m stride, or the box's fastest index
   const bool to = op.box_class ? op.box_class != 1 : (x.c_ms0 == 1 && x.c_ns0 != 1);
   const dim3 grid((unsigned)tiles, (unsigned)op.batch, 1);
-  if (op.f32) {
+  if (op.f64) {
+    ECG_CHECK(op.C != nullptr, "gemm: a packed FP64 product writes an FP64 output");
+    if (to) hipLaunchKernelGGL((gemm_f64p_dma_kernel<true, true>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_f64p_dma_kernel<false, true>), grid, dim3(512), 0, stream, x);
+  } else if (op.f32) {
     if (to) hipLaunchKernelGGL((gemm_f32p_dma_kernel<true, true>), grid, dim3(512), 0, stream, x);
     else hipLaunchKernelGGL((gemm_f32p_dma_kernel<false, true>), grid, dim3(512), 0, stream, x);
   } else if (bf16_kernel == 1) {

@@ -349,15 +349,24 @@ void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const do
 struct Planes {
   DevBuf* buf = nullptr;     // storage owned by the context (grown on demand)
   bool f32 = false;          // one FP32 plane (f) instead of BF16 hi / lo
+  bool f64 = false;          // one FP64 plane (d, pack_f64p_kernel layout)
   uint16_t* hi = nullptr;
   uint16_t* lo = nullptr;
   float* f = nullptr;
+  double* d = nullptr;
   long ld = 0;
   // rows of src as in GemmEngine::pack_rows_bf16x3; 256 readable rows are kept after the last
   void make(GemmEngine& g, const double* src, long rows, int d0, long s0, long s1, int K, long sk) {
     ld = ((K + 31) / 32) * 32;
     const long rows_alloc = ((rows + 63) / 64) * 64 + 256;
     const long packed = ((rows + 63) / 64) * 64;
+    if (f64) {
+      buf->ensure(sizeof(double) * rows_alloc * ld);
+      d = static_cast<double*>(buf->p);
+      ECG_HIP(hipMemsetAsync(d + packed * ld, 0, sizeof(double) * (rows_alloc - packed) * ld, g.stream));
+      g.pack_rows_f64(src, rows, d0, s0, s1, K, sk, (int)ld, d);
+      return;
+    }
     buf->ensure(sizeof(uint16_t) * 2 * rows_alloc * ld);   // 4 bytes per element either way
     if (f32) {
       f = static_cast<float*>(buf->p);
@@ -375,24 +384,27 @@ struct Planes {
 struct PackedInts { Planes W, Wt, Hn; };
 struct PackedAmps { Planes T3, T4, Ts; };
 
-void pack_ints(GemmEngine& g, int v, int o, const TripleInts& I, bool f32, PackedInts& P) {
+// plane kind: 0 BF16 hi/lo, 1 FP32, 2 FP64
+void pack_ints(GemmEngine& g, int v, int o, const TripleInts& I, int kind, PackedInts& P) {
   const long vv = (long)v * v, vvv = vv * v;
-  P.W.f32 = P.Wt.f32 = P.Hn.f32 = f32;
+  P.W.f32 = P.Wt.f32 = P.Hn.f32 = (kind == 1);
+  P.W.f64 = P.Wt.f64 = P.Hn.f64 = (kind == 2);
   P.W.make(g, I.W, (long)o * vv, (int)vv, 1, vvv, v, vv);
   P.Wt.make(g, I.Wt, (long)o * vv, (int)vv, 1, vvv, v, vv);
   P.Hn.make(g, I.Hn, (long)o * o * v, v, 1, (long)v * o, o, v);
 }
 
-void pack_amps(GemmEngine& g, int v, int o, const double* T, bool f32, PackedAmps& P) {
+void pack_amps(GemmEngine& g, int v, int o, const double* T, int kind, PackedAmps& P) {
   const long vv = (long)v * v;
-  P.T3.f32 = P.T4.f32 = P.Ts.f32 = f32;
+  P.T3.f32 = P.T4.f32 = P.Ts.f32 = (kind == 1);
+  P.T3.f64 = P.T4.f64 = P.Ts.f64 = (kind == 2);
   P.T3.make(g, T, (long)o * vv, 1 << 30, 1, 0, o, vv * o);
   P.T4.make(g, T, (long)o * vv, (int)vv, 1, vv * o, o, vv);
   P.Ts.make(g, T, (long)o * o * v, v, 1, vv, v, v);
 }
 
-// K[a,b,c] (FP32) for the triplets (i0 + z, j, k), z < nb, from packed planes (BF16 hi/lo
-// or FP32 products): the
+// K[a,b,c] (FP32; FP64 from FP64 planes) for the triplets (i0 + z, j, k), z < nb, from
+// packed planes (BF16 hi/lo, FP32 or FP64 products): the
 // same twelve products as build_K, each class with the pair of K indices carried
 // by the integral (or amplitude pair) operand as M (v^2 rows) and the single index
 // as N (v), so all three launches have the shape v^2 x v x (2v + 2o), batched over
@@ -400,14 +412,17 @@ void pack_amps(GemmEngine& g, int v, int o, const double* T, bool f32, PackedAmp
 // combine's unit), one contiguous 512-element run per box; member z's parts start
 // kz elements after member z-1's.
 void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, const PackedAmps& T,
-                    const PackedInts& I, float* const K[3], long kz) {
+                    const PackedInts& I, float* const K[3], double* const Kd[3], long kz) {
   const long vv = (long)v * v;
   auto rowsW = [&](const Planes& P, int m) { return m * vv * P.ld; };           // block m of a v^2-row family
   auto rowsS = [&](const Planes& P, int p, int q) { return (p + (long)o * q) * v * P.ld; };   // v-row block
   // operand offsets as functions of i; the batch advances them by their change per unit i
   auto seg = [&](PackedSeg& s, const Planes& A, auto aoff, const Planes& B, auto boff) {
     const long a0 = aoff(i0), b0 = boff(i0);
-    if (A.f32) {   // FP32 planes: the plane pointers travel in Ah / Bh (PackedOp::f32)
+    if (A.f64) {   // FP64 planes: the plane pointers travel in Ah / Bh (PackedOp::f64)
+      s.Ah = reinterpret_cast<const uint16_t*>(A.d + a0); s.Al = nullptr;
+      s.Bh = reinterpret_cast<const uint16_t*>(B.d + b0); s.Bl = nullptr;
+    } else if (A.f32) {   // FP32 planes: the plane pointers travel in Ah / Bh (PackedOp::f32)
       s.Ah = reinterpret_cast<const uint16_t*>(A.f + a0); s.Al = nullptr;
       s.Bh = reinterpret_cast<const uint16_t*>(B.f + b0); s.Bl = nullptr;
     } else {
@@ -420,7 +435,13 @@ void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, c
   };
   PackedOp op;
   op.f32 = T.Ts.f32;
-  op.M = (int)vv; op.N = v; op.nseg = 4; op.Cf = K[0];
+  op.f64 = T.Ts.f64;
+  auto out = [&](int q) {
+    op.C = op.f64 ? Kd[q] : nullptr;
+    op.Cf = op.f64 ? nullptr : K[q];
+  };
+  op.M = (int)vv; op.N = v; op.nseg = 4;
+  out(0);
   op.algorithmic_k = 2LL * v + 2LL * o;
   op.batch = nb; op.c_z = kz;
   // class a: m = (b,c), n = a
@@ -438,7 +459,7 @@ void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, c
   seg(op.seg[1], I.Wt, [&](int i) { return rowsW(I.Wt, i); }, T.Ts, [&](int) { return rowsS(T.Ts, j, k); });
   seg(op.seg[2], T.T3, [&](int i) { return rowsW(T.T3, i); }, I.Hn, [&](int) { return rowsS(I.Hn, k, j); });
   seg(op.seg[3], T.T4, [&](int) { return rowsW(T.T4, k); }, I.Hn, [&](int i) { return rowsS(I.Hn, i, j); });
-  op.box_class = 2; op.Cf = K[1];
+  op.box_class = 2; out(1);
   g.run_bf16x3_packed(op);
   // class c: m = (a,b), n = c; terms 4: W_j[a,b,d] T^{ki}[c,d], 6: W_i[b,a,d] T^{kj}[c,d],
   //       7: T[a,b,i,l] (-H[l,c,j,k]), 8: T[a,b,l,j] (-H[l,c,i,k])
@@ -446,7 +467,7 @@ void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, c
   seg(op.seg[1], I.Wt, [&](int i) { return rowsW(I.Wt, i); }, T.Ts, [&](int) { return rowsS(T.Ts, k, j); });
   seg(op.seg[2], T.T3, [&](int i) { return rowsW(T.T3, i); }, I.Hn, [&](int) { return rowsS(I.Hn, j, k); });
   seg(op.seg[3], T.T4, [&](int) { return rowsW(T.T4, j); }, I.Hn, [&](int i) { return rowsS(I.Hn, i, k); });
-  op.box_class = 3; op.Cf = K[2];
+  op.box_class = 3; out(2);
   g.run_bf16x3_packed(op);
 }
 
@@ -531,9 +552,12 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
       IL = TripleInts{vvvoL.d(), vvvoLT.d(), ovooLN.d()};
     }
   }
-  // BF16 hi/lo and FP32 products run on operands packed once for all triplets, with K in FP32
-  const bool packed = (pr == Prec::BF16X3 || pr == Prec::F32);
+  // BF16 hi/lo and FP32 products run on operands packed once for all triplets, with K in FP32;
+  // FP64 ones on FP64 planes with K in FP64 (unless the engine's FP64 plane kernel is off)
+  const bool f64p = (pr == Prec::F64 && ops.gemm.f64_kernel == 1);
+  const bool packed = (pr == Prec::BF16X3 || pr == Prec::F32 || f64p);
   const bool f32 = (pr == Prec::F32);
+  const int kind = f32 ? 1 : (f64p ? 2 : 0);
   PackedInts PI, PIL;
   PackedAmps PT, PU;
   PI.W.buf = &W[11]; PI.Wt.buf = &W[12]; PI.Hn.buf = &W[13];
@@ -541,11 +565,11 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   PU.T3.buf = &W[17]; PU.T4.buf = &W[18]; PU.Ts.buf = &W[19];
   PIL.W.buf = &W[20]; PIL.Wt.buf = &W[21]; PIL.Hn.buf = &W[22];
   if (packed) {
-    pack_ints(ops.gemm, v, o, IT, f32, PI);
-    pack_amps(ops.gemm, v, o, dT2.d(), f32, PT);
+    pack_ints(ops.gemm, v, o, IT, kind, PI);
+    pack_amps(ops.gemm, v, o, dT2.d(), kind, PT);
     if (lambda) {
-      pack_amps(ops.gemm, v, o, dU2.d(), f32, PU);
-      if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, f32, PIL);
+      pack_amps(ops.gemm, v, o, dU2.d(), kind, PU);
+      if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, kind, PIL);
     }
   }
   const PackedInts& PIU = (lambda && !c.hermitian) ? PIL : PI;
@@ -555,7 +579,7 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   const int BMAX = packed ? 8 : 1;
   // the packed path stores K in whole 8x8x8 boxes (ntile^3 of them, padding zeroed below)
   const long kmem = packed ? nblocks * TT * TT * TT : vvv;   // elements of one part of one triplet
-  const size_t kbytes = (packed ? sizeof(float) : sizeof(double)) * kmem * BMAX;   // one part, all members
+  const size_t kbytes = ((packed && !f64p) ? sizeof(float) : sizeof(double)) * kmem * BMAX;   // one part, all members
   const long pz = nblocks * NPART;
   DevBuf& K = W[23];
   DevBuf& KU = W[24];
@@ -625,7 +649,7 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   CombArgs<float> cf{};
   cf.nv = v; cf.no = o; cf.ntile = ntile; cf.tri = ca.tri;
   cf.eo = ca.eo; cf.ev = ca.ev; cf.W = ca.W; cf.Y = ca.Y; cf.part = ca.part;
-  ca.boxed = 0;
+  ca.boxed = f64p ? 1 : 0;
   cf.boxed = 1;
   ca.kz = kmem; cf.kz = kmem;
   ca.pz = pz; cf.pz = pz;
@@ -640,10 +664,15 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   // ordered accumulation into IntX, IntY and the energy
   auto run_block = [&](int i0, int nb, int j, int k, bool reduce) {
     if (packed) {
-      build_K_packed(ops.gemm, v, o, i0, nb, j, k, PT, PI, Kf, kmem);
-      if (lambda) build_K_packed(ops.gemm, v, o, i0, nb, j, k, PU, PIU, KUf, kmem);
-      cf.i = i0; cf.j = j; cf.k = k;
-      hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, cf);
+      build_K_packed(ops.gemm, v, o, i0, nb, j, k, PT, PI, Kf, Kd, kmem);
+      if (lambda) build_K_packed(ops.gemm, v, o, i0, nb, j, k, PU, PIU, KUf, KUd, kmem);
+      if (f64p) {
+        ca.i = i0; ca.j = j; ca.k = k;
+        hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, ca);
+      } else {
+        cf.i = i0; cf.j = j; cf.k = k;
+        hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, cf);
+      }
     } else {
       build_K(ops.gemm, pr, v, o, i0, j, k, dT2.d(), IT, Kd);
       if (lambda) build_K(ops.gemm, pr, v, o, i0, j, k, dU2.d(), IL, KUd);
@@ -661,7 +690,7 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   const long nrange = count < 0 ? ntrip_all : std::min<long>(count, std::max<long>(0, ntrip_all - first));
   int nbatch = 1;
   if (packed) {
-    int& tuned = c.triples_batch[f32 ? 1 : 0][lambda ? 1 : 0];
+    int& tuned = c.triples_batch[kind][lambda ? 1 : 0];
     const long tiles = (((long)v * v + 255) / 256) * ((v + 127) / 128);
     nbatch = (int)std::max<long>(1, std::min<long>(BMAX, 1024 / std::max<long>(1, tiles)));
     if (tuned > 0) {
