@@ -2,16 +2,20 @@
 """Benchmark of the closed-shell CCSD iteration and (T) on one MI355X.
 
 Workload (default): o=40, v=360, naux=1200, seed 2 -- the seeded synthetic
-density-fitted system the headline metric is quoted on.  A step is one
-device-resident CCSD iteration (residual, Hylleraas energy, amplitude update,
-DIIS) with integrals and amplitudes resident in HBM before the timed region.
+density-fitted system the headline metric is quoted on, with the DF factor
+width at which CCSD converges (conditioned_scale; the literal 0.05 width
+diverges at these sizes, the shapes and the work per iteration are the same).
+A step is one device-resident CCSD iteration (residual, Hylleraas energy,
+amplitude update, DIIS) with integrals and amplitudes resident in HBM before
+the timed region; every warm-up and timed state is checked to be finite.
 
 Prints one JSON line.  value = algorithmic TFLOP/s of the FP64 CCSD iteration
 (count in elemco_amd/flops.py); ms_per_step = FP64 time per iteration.  The
 same line carries the mixed-precision iterations (FP32 and BF16 hi/lo
-products), full FP64 and mixed solves with their energy difference, (T) rates
-on a sample of triplets, the roofline of the dominant kernel and the CPU
-baseline.
+products), full FP64 and mixed solves with their energy difference, the
+complete (T) per precision on converged amplitudes of the o=40 v=360
+naux=1200 seed-4 system, the 64-system batch, the roofline of the dominant
+kernel and the CPU baseline.
 """
 import argparse
 import json
@@ -23,7 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "elemco.jl_amd"))
 
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # dense MFMA TFLOP/s of MI355X
-CONFIGS = {"c2": (40, 360, 1200, 2), "c1": (20, 180, 600, 1), "c3": (30, 270, 900, 3), "c0": (5, 19, 72, 0)}
+CONFIGS = {"c2": (40, 360, 1200, 2), "c1": (20, 180, 600, 1), "c3": (30, 270, 900, 3), "c0": (5, 19, 72, 0),
+           "c4a": (40, 360, 1200, 4)}
 
 
 def log(*a):
@@ -32,8 +37,8 @@ def log(*a):
 
 def cpu_baseline(sample=(20, 180, 600, 1)):
     """Time the CPU oracle (numpy/OpenBLAS restatement) on CCSD residuals of a bounded sample:
-    two residuals of the o=20 v=180 naux=600 system (about 6 s each on 16 host threads; the
-    o=40 v=360 residual would take minutes)."""
+    two residuals of the o=20 v=180 naux=600 system (about 6 s each on 16 host threads; one
+    o=40 v=360 residual would take about 5 minutes)."""
     sys.path.insert(0, ROOT)
     from oracle.ccsd import ECInfo, calc_fock_matrix, calc_MP2, calc_cc_resid
     from oracle.synthetic import make_df_system
@@ -53,9 +58,10 @@ def cpu_baseline(sample=(20, 180, 600, 1)):
         calc_cc_resid(EC, EC.files["T_vo"], EC.files["T_vvoo"])
     dt = (time.perf_counter() - t0) / reps
     return {"value": round(ccsd_iteration(o, v, q) / dt / 1e12, 5), "unit": "TFLOP/s", "cores": int(cores),
-            "kind": "port",
+            "host_cpus_available": len(os.sched_getaffinity(0)), "kind": "port",
             "sample": f"{reps} CCSD residuals of the numpy/OpenBLAS oracle at o={o} v={v} naux={q} seed={seed} "
-                      f"({dt:.1f} s each); rate uses the same algorithmic flop count as the GPU figure"}
+                      f"({dt:.1f} s each; the GPU figure is at o=40 v=360 naux=1200), on {cores} BLAS threads; "
+                      f"rate uses the same algorithmic flop count as the GPU figure"}
 
 
 def measured_traffic(config):
@@ -74,35 +80,42 @@ def measured_traffic(config):
 
 
 def time_ccsd(E, ctx, meth, prec, steps, warmup):
+    """W untimed then K timed device-resident iterations from the MP2 guess of the conditioned
+    system (every state finite: the run stops with an error otherwise).  Returns (seconds per
+    step, stats, info of the last step, [E_H, NormR] of the first and last timed steps)."""
+    import math
     opts = E.default_opts()
     ctx.state_init(opts=opts)
+
+    def check(info, what):
+        if not all(math.isfinite(x) for x in info[:7]):
+            raise SystemExit(f"bench: non-finite CCSD state at {what}: {list(info)}")
+
     for k in range(warmup):
         t = time.time()
         info = ctx.state_step(meth, prec, opts)
-        log(f"  warmup {k}: {time.time() - t:.3f} s  E={info[0]:.10g}")
+        check(info, f"warm-up step {k}")
+        log(f"  warmup {k}: {time.time() - t:.3f} s  E={info[0]:.12f} NormR={info[5]:.3e}")
     ctx.profile(True)
+    infos = []
     t0 = time.perf_counter()
     for k in range(steps):
-        info = ctx.state_step(meth, prec, opts)
+        infos.append(ctx.state_step(meth, prec, opts))
     dt = time.perf_counter() - t0
     st = ctx.stats()
     ctx.profile(False)
-    return dt / steps, st, info
+    for k, info in enumerate(infos):
+        check(info, f"timed step {k}")
+    log(f"  timed: E {infos[0][0]:.12f} -> {infos[-1][0]:.12f}, NormR {infos[0][5]:.3e} -> {infos[-1][5]:.3e}")
+    trace = {"first": [float(infos[0][0]), float(infos[0][5])], "last": [float(infos[-1][0]), float(infos[-1][5])]}
+    return dt / steps, st, infos[-1], trace
 
 
-def solve_study(E, o, v, q, seed, meth, maxit=60):
-    """Full solves from the MP2 guess, FP64 and mixed, on the conditioned system of the same sizes.
-
-    The literal N(0, 0.05^2) factors give CCSD iterations that diverge at these
-    sizes, so converged energies are compared on the variant whose factor width
-    is scaled by conditioned_scale(o, v, q); the shapes (and the per-iteration
-    cost) are those of the timed workload.  Convergence: the reference's test at
-    thr = 1e-10 (squared residual norm), always confirmed on an FP64 iteration.
-    """
-    from elemco_amd.synthetic import conditioned_scale
-    sc = conditioned_scale(o, v, q)
-    EC = E.ECInfo.synthetic(o, v, q, seed, scale=sc)
-    out = {"system": f"o={o} v={v} naux={q} seed={seed}, factor width 0.05*{sc:.4f}", "thr": 1e-10}
+def solve_study(E, EC, desc, meth, maxit=60):
+    """Full solves from the MP2 guess, FP64 and mixed (defect-corrected), on the conditioned
+    system of the benchmark sizes.  Convergence: the reference's test at thr = 1e-10 (squared
+    residual norm), always confirmed on an FP64 iteration."""
+    out = {"system": desc, "thr": 1e-10}
     res = {}
     for name, prec in (("fp64", E.ECG_FP64), ("fp32", E.ECG_FP32), ("bf16x3", E.ECG_BF16X3)):
         opts = E.default_opts(maxit=maxit)
@@ -111,13 +124,12 @@ def solve_study(E, o, v, q, seed, meth, maxit=60):
         dt = time.perf_counter() - t0
         st = EC.ctx.stats()
         res[name] = {"E_corr": e[0], "iterations": it, "converged": conv, "fp64_iterations": st["fp64_iterations"],
-                     "seconds": round(dt, 2), "s_per_iteration": round(dt / max(1, it), 4)}
+                     "final_norm_r": e[5], "seconds": round(dt, 2), "s_per_iteration": round(dt / max(1, it), 4)}
         log(f"  {name}: E={e[0]:.12f} it={it} conv={conv} {dt:.1f} s")
     for name in ("fp32", "bf16x3"):
         res[name]["abs_dE_vs_fp64"] = abs(res[name]["E_corr"] - res["fp64"]["E_corr"])
         res[name]["time_to_solution_speedup"] = round(res["fp64"]["seconds"] / res[name]["seconds"], 3)
     out.update(res)
-    EC.ctx.close()
     return out
 
 
@@ -249,24 +261,31 @@ def gradient_check(E, o, v, q, seed, meth, ndir=20, step=1e-4, thr=1e-20):
 
 
 def triples_energy(E, o, v, q, seed, meth):
-    """Full (T) per precision on converged amplitudes of the conditioned system of the given sizes.
+    """Complete (T) per precision on converged amplitudes of the conditioned system of the given
+    sizes (the c4a workload at o=40 v=360 naux=1200 seed 4).
 
-    The amplitudes come from a defect-corrected BF16x3 solve (it ends at the FP64
-    fixed point, thr 1e-10); every (T) then runs over all triplets, so the energy
-    differences are the complete error of the low-precision products."""
+    The amplitudes come from a defect-corrected BF16x3 solve (it ends at the FP64 fixed
+    point, thr 1e-10); every (T) then runs over all triplets, so the energy differences are
+    the complete error of the low-precision products.  Rates use the algorithmic count
+    (o(o+1)(o+2)/6 - o)(12 v^4 + 12 o v^3)."""
+    from elemco_amd.flops import triples
     from elemco_amd.synthetic import conditioned_scale
     sc = conditioned_scale(o, v, q)
     EC = E.ECInfo.synthetic(o, v, q, seed, scale=sc)
-    out = {"system": f"o={o} v={v} naux={q} seed={seed}, factor width 0.05*{sc:.4f}"}
+    out = {"system": f"o={o} v={v} naux={q} seed={seed}, factor width 0.05*{sc:.4f}",
+           "algorithmic_tflop": round(triples(o, v) / 1e12, 2)}
     t0 = time.perf_counter()
     e, T1, T2, it, conv = EC.ctx.cc_solve(meth, E.ECG_BF16X3, E.default_opts(maxit=60))
     out["cc_solve"] = {"E_corr": e[0], "iterations": it, "converged": conv, "seconds": round(time.perf_counter() - t0, 2)}
     log(f"  solve: E={e[0]:.12f} it={it} conv={conv}")
+    if not conv:
+        raise SystemExit("bench: the (T) workload's CCSD solve did not converge")
     for name, prec in (("fp64", E.ECG_FP64), ("fp32", E.ECG_FP32), ("bf16x3", E.ECG_BF16X3)):
         t0 = time.perf_counter()
         et, etb = EC.ctx.pert_triples(T1, T2, precision=prec)
         dt = time.perf_counter() - t0
-        out[name] = {"E_T": et, "E_T_bracket": etb, "seconds": round(dt, 2)}
+        out[name] = {"E_T": et, "E_T_bracket": etb, "seconds": round(dt, 2),
+                     "tflops": round(triples(o, v) / dt / 1e12, 2), "triplets_per_block": EC.ctx.stats()["triples_block"]}
         log(f"  (T) {name}: {et:.12f} [T] {etb:.12f} {dt:.1f} s")
     for name in ("fp32", "bf16x3"):
         out[name]["abs_dE_T_vs_fp64"] = abs(out[name]["E_T"] - out["fp64"]["E_T"])
@@ -326,7 +345,9 @@ def main():
     ap.add_argument("--config", default=None, choices=sorted(CONFIGS),
                     help="workload sizes (default c2; c3 with --gradcheck)")
     ap.add_argument("--method", default="ccsd", choices=["ccsd", "dcsd"])
-    ap.add_argument("--triplets", type=int, default=48, help="(T) sample size; 0 skips (T)")
+    ap.add_argument("--triplets", type=int, default=48,
+                    help="(T) sample size of --triples-only; 0 skips the complete (T) of the default line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-system batch of the default line")
     ap.add_argument("--no-mixed", action="store_true", help="time only the FP64 iteration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--engine", action="store_true",
@@ -371,40 +392,55 @@ def main():
         log(f"gradient check o={o} v={v} naux={q}")
         print(json.dumps(gradient_check(E, o, v, q, seed, meth, args.directions)), flush=True)
         return
-    t0 = time.time()
-    EC = E.ECInfo.synthetic(o, v, q, seed)
-    log(f"setup o={o} v={v} naux={q}: {time.time() - t0:.1f} s")
-    ctx = EC.ctx
     if args.triples_only:
+        t0 = time.time()
+        EC = E.ECInfo.synthetic(o, v, q, seed)
+        log(f"setup o={o} v={v} naux={q}: {time.time() - t0:.1f} s")
         out = {"workload": f"(T) on MP2 amplitudes, o={o} v={v} naux={q} seed={seed}"}
         for name, prec in (("fp64", E.ECG_FP64), ("fp32", E.ECG_FP32), ("bf16x3", E.ECG_BF16X3)):
             log(f"(T) {name}")
-            out[name] = time_triples(E, ctx, prec, args.triplets, o, v)
+            out[name] = time_triples(E, EC.ctx, prec, args.triplets, o, v)
         out["bf16x3"]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_triplet"] / out["bf16x3"]["ms_per_triplet"], 3)
         out["fp32"]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_triplet"] / out["fp32"]["ms_per_triplet"], 3)
         print(json.dumps(out), flush=True)
         return
+    from elemco_amd.synthetic import conditioned_scale
+    sc = conditioned_scale(o, v, q)
+    t0 = time.time()
+    EC = E.ECInfo.synthetic(o, v, q, seed, scale=sc)
+    desc = f"o={o} v={v} naux={q} seed={seed}, factor width 0.05*{sc:.4f}"
+    log(f"setup {desc}: {time.time() - t0:.1f} s")
+    ctx = EC.ctx
     flops = ccsd_iteration(o, v, q, dcsd=(args.method == "dcsd"))
 
     log("FP64 iterations")
-    s64, st, info = time_ccsd(E, ctx, meth, E.ECG_FP64, args.steps, args.warmup)
+    s64, st, info, trace = time_ccsd(E, ctx, meth, E.ECG_FP64, args.steps, args.warmup)
     gemm_avg_ms = st["gemm_ms"] / max(1, st["gemm_launches"])
     launches = st["gemm_launches"] / args.steps
     achieved = (flops / launches) / (gemm_avg_ms * 1e-3) / 1e12
+    traffic = measured_traffic(args.config)
     line = {
         "metric": "s/CCSD-iter + (T) TFLOP/s, mixed vs FP64 GPU at |dE|<=1e-6 Eh, o40 v360",
         "value": round(flops / s64 / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(s64 * 1e3, 2), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded DF integrals, splitmix64; no checkpoints)",
-        "config": {"workload": f"closed-shell {args.method.upper()} iteration, o={o} v={v} naux={q} seed={seed}",
+        "config": {"workload": f"closed-shell {args.method.upper()} iteration from the MP2 guess (warm-up "
+                               f"iterations untimed, every state finite), o={o} v={v} naux={q} seed={seed}, DF "
+                               f"factor width 0.05*{sc:.4f} (the width at which CCSD converges; the literal "
+                               f"0.05 diverges at these sizes -- shapes and work per iteration are the same)",
                    "nocc": o, "nvirt": v, "naux": q, "method": args.method, "precision": "fp64",
-                   "algorithmic_tflop_per_step": round(flops / 1e12, 3)},
+                   "algorithmic_tflop_per_step": round(flops / 1e12, 3),
+                   "timed_energy_and_norm_r": trace},
         "roofline": {"bound": "mfma",
                      "kernel": "FP64 MFMA GEMM launches: ecg::gemm_kernel<double,...> and, for products with "
                                "M N / (M + N) >= 1024, ecg::gemm_f64p_dma_kernel (its operand packing runs as "
                                "separate HBM-bound pack_f64p_kernel launches, not counted here)",
                      "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp64"], 4), "traffic": measured_traffic(args.config),
+                     "frac": round(achieved / PEAK["fp64"], 4), "traffic": traffic,
+                     "traffic_source": ("not measured in this run: HBM bytes per FP64 GEMM launch read from the "
+                                        "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary "
+                                        "profiles/ccsd_iteration_o40_v360_naux1200_fp64_traffic.md")
+                     if traffic is not None else None,
                      "launches_per_step": launches, "avg_launch_ms": round(gemm_avg_ms, 4),
                      "algorithmic_bytes_per_launch": round(st["gemm_bytes"] / max(1, st["gemm_launches"])),
                      "gemm_ms_per_step": round(st["gemm_ms"] / args.steps, 2),
@@ -417,7 +453,7 @@ def main():
                 ("bf16x3", E.ECG_BF16X3, "BF16 hi/lo products (3 BF16 MFMA terms), FP32 accumulation",
                  PEAK["bf16"] / 3.0)):
             log(f"{name} iterations")
-            sl, stl, _ = time_ccsd(E, ctx, meth, prec, args.steps, args.warmup)
+            sl, stl, _, tr = time_ccsd(E, ctx, meth, prec, args.steps, args.warmup)
             per_launch = (flops / (stl["gemm_launches"] / args.steps)) / (
                 stl["gemm_ms"] / max(1, stl["gemm_launches"]) * 1e-3) / 1e12
             line["mixed"][name] = {
@@ -425,17 +461,18 @@ def main():
                 "ms_per_step": round(sl * 1e3, 2), "tflops": round(flops / sl / 1e12, 3),
                 "speedup_vs_fp64": round(s64 / sl, 3),
                 "gemm_achieved_tflops": round(per_launch, 3),
-                "gemm_peak_tflops": round(peak, 1)}
-    if args.triplets > 0:
-        log("(T) samples")
-        line["triples"] = {"fp64": time_triples(E, ctx, E.ECG_FP64, args.triplets, o, v)}
-        if not args.no_mixed:
-            line["triples"]["fp32"] = time_triples(E, ctx, E.ECG_FP32, args.triplets, o, v)
-            line["triples"]["bf16x3"] = time_triples(E, ctx, E.ECG_BF16X3, args.triplets, o, v)
+                "gemm_peak_tflops": round(peak, 1), "timed_energy_and_norm_r": tr}
     if not args.no_solve:
         log("full solves, FP64 and mixed")
-        ctx.close()
-        line["solve"] = solve_study(E, o, v, q, seed, meth)
+        line["solve"] = solve_study(E, EC, desc, meth)
+    ctx.close()
+    if args.triplets > 0:
+        to, tv, tq, ts = CONFIGS["c4a"] if args.config == "c2" else (o, v, q, seed)
+        log(f"complete (T) per precision o={to} v={tv} naux={tq} seed={ts}")
+        line["triples"] = triples_energy(E, to, tv, tq, ts, meth)
+    if not args.no_batch:
+        log("64-system batch")
+        line["batch"] = batch_study(E)
     if not args.no_cpu and args.config != "c0":
         log("CPU baseline")
         try:

@@ -168,6 +168,35 @@ int ecg_fcidump_read(const char* path, ecg_fcidump** out) {
   });
 }
 
+int ecg_fcidump_from_integrals(int norb, int nelec, int ms2, int st, const double* h, const double* eri, double e0,
+                               ecg_fcidump** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  return host_guarded([&] {
+    ECG_CHECK(norb > 0 && h && eri, "norb must be positive and h, eri given");
+    ECG_CHECK(nelec >= 0 && ms2 >= 0 && st >= 0, "NELEC, MS2 and ST must be non-negative");
+    auto f = std::make_unique<ecg_fcidump>();
+    ecg::Fcidump& fd = f->fd;
+    const size_t n = (size_t)norb;
+    fd.norb = norb;
+    fd.int0 = e0;
+    fd.h.assign(h, h + n * n);
+    fd.eri.assign(eri, eri + n * n * n * n);
+    auto put = [&](const std::string& key, std::vector<std::string> vals) {
+      fd.head[key] = std::move(vals);
+      fd.head_type[key] = 'i';
+    };
+    put("NORB", {std::to_string(norb)});
+    put("NELEC", {std::to_string(nelec)});
+    put("MS2", {std::to_string(ms2)});
+    put("ISYM", {"1"});
+    put("ORBSYM", std::vector<std::string>(n, "1"));
+    if (st > 0) put("ST", {std::to_string(st)});
+    *out = f.release();
+    return 0;
+  });
+}
+
 int ecg_fcidump_free(ecg_fcidump* fd) {
   if (!fd) return -1;
   delete fd;
@@ -540,7 +569,18 @@ int ecg_energy_grad(ecg_ctx* ctx, int method, const double* T1, const double* T2
   return guarded(ctx, [&](ecg::Context& c) {
     check_method(method);
     ECG_CHECK(T1 && T2 && U1 && U2, "NULL pointer");
-    ecg::energy_grad(c, method, T1, T2, U1, U2, dE_dh, dE_dB);
+    ECG_CHECK(c.hermitian, "ecg_energy_grad: two-sided DF factors (BR given): use ecg_energy_grad_lr");
+    ecg::energy_grad(c, method, T1, T2, U1, U2, dE_dh, dE_dB, nullptr);
+  });
+}
+
+int ecg_energy_grad_lr(ecg_ctx* ctx, int method, const double* T1, const double* T2, const double* U1,
+                       const double* U2, double* dE_dh, double* dE_dBL, double* dE_dBR) {
+  return guarded(ctx, [&](ecg::Context& c) {
+    check_method(method);
+    ECG_CHECK(T1 && T2 && U1 && U2, "NULL pointer");
+    ECG_CHECK(!c.hermitian, "ecg_energy_grad_lr: the context has one DF factor (BR = NULL): use ecg_energy_grad");
+    ecg::energy_grad(c, method, T1, T2, U1, U2, dE_dh, dE_dBL, dE_dBR);
   });
 }
 

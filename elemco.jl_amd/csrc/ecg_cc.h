@@ -62,6 +62,7 @@ struct Context {
   // energies
   double hf_energy();
   void mp2_guess(double* T1d, double* T2d, double* emp2 /*[E,ESS,EOS]*/);
+  void mp2_amplitudes(double* T1d, double* T2d);       // the guess' amplitudes only (kernels, no allocation)
   void doubles_energy(const double* T2d, double* e /*[E,ESS,EOS]*/);
   void singles_energy(const double* T1d, bool fock_only, double* e);
   void hylleraas(const double* T1d, const double* T2d, const double* R1d, const double* R2d,
@@ -115,5 +116,9 @@ struct Context {
   void residual_graph(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d);
   ~Context();
 };
+
+// Per-(i,j) block sums of two [a,b,i,j] tensors (v = nv, npairs = o^2 blocks):
+// part[2 p] = sum_ab X[a,b,p] Y[a,b,p],  part[2 p + 1] = sum_ab X[b,a,p] Y[a,b,p].
+void launch_pair_dots(hipStream_t s, int nv, long npairs, const double* X, const double* Y, double* part);
 
 }  // namespace ecg

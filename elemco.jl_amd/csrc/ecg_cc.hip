@@ -545,8 +545,7 @@ void Context::residual_graph(int method, Prec pr, const double* T1d, const doubl
 void Context::doubles_de(const double* X, const double* Y, double& d, double& e) {
   long np = (long)no * no;
   dots_part.ensure(sizeof(double) * 2 * np);
-  hipLaunchKernelGGL(cs_dots_kernel, dim3((unsigned)np), dim3(256), 0, stream, nv, X, Y, dots_part.d());
-  ECG_HIP(hipGetLastError());
+  launch_pair_dots(stream, nv, np, X, Y, dots_part.d());
   dots_host.resize(2 * np);
   ECG_HIP(hipMemcpyAsync(dots_host.data(), dots_part.p, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, stream));
   ECG_HIP(hipStreamSynchronize(stream));
@@ -624,8 +623,8 @@ void Context::update(double* T1d, double* T2d_, const double* R1d, const double*
   ECG_HIP(hipGetLastError());
 }
 
-// MP2 guess (cc.jl:1011-1024): T2 = -<ab|ij>/D, T1 = -f_vo/(e_a - e_i)
-void Context::mp2_guess(double* T1d, double* T2d_, double* emp2) {
+// MP2 amplitudes (cc.jl:1011-1024): T2 = -<ab|ij>/D, T1 = -f_vo/(e_a - e_i); kernels only
+void Context::mp2_amplitudes(double* T1d, double* T2d_) {
   const double* eo = eps.d();
   const double* ev = eps.d() + no;
   hipLaunchKernelGGL(update_doubles_kernel, dim3(nblk(nvvoo())), dim3(256), 0, stream, nv, no, eo, ev, 0.0,
@@ -633,6 +632,17 @@ void Context::mp2_guess(double* T1d, double* T2d_, double* emp2) {
   hipLaunchKernelGGL(update_singles_kernel, dim3((unsigned)((nvo() + 255) / 256)), dim3(256), 0, stream, nv, no, eo,
                      ev, 0.0, fock.d() + no, (long)n, T1d, 1);
   ECG_HIP(hipGetLastError());
+}
+
+// Per-(i,j) partial sums [d, e] of two v x v x o x o tensors (pair_dots in ecg_cc.h)
+void launch_pair_dots(hipStream_t s, int nv, long npairs, const double* X, const double* Y, double* part) {
+  hipLaunchKernelGGL(cs_dots_kernel, dim3((unsigned)npairs), dim3(256), 0, s, nv, X, Y, part);
+  ECG_HIP(hipGetLastError());
+}
+
+// MP2 guess (cc.jl:1011-1024) and its energy
+void Context::mp2_guess(double* T1d, double* T2d_, double* emp2) {
+  mp2_amplitudes(T1d, T2d_);
   double e2[3], e1[3];
   doubles_energy(T2d_, e2);
   singles_energy(T1d, true, e1);

@@ -434,10 +434,15 @@ typedef __attribute__((address_space(3))) void* lds_vptr;
 // the wave).  Issued as inline assembly: the compiler then neither tracks these loads
 // (the kernel waits for them with explicit vmcnt waits) nor assumes they may alias
 // every LDS read that follows (which made it wait for all loads in flight).  One
-// wait state separates the M0 write from the load that reads it.
+// wait state separates the M0 write from the load that reads it.  M0 is named in the
+// clobber list; the AMDGPU backend treats it as a reserved register (it never keeps a
+// value of its own there and warns about the clobber, silenced below), and in the built
+// kernels every M0 write is one of these.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void dma16(const void* src, lds_vptr dst) {
   const unsigned m0 = (unsigned)(__UINTPTR_TYPE__)dst;
-  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory");
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m0), "v"(src) : "memory", "m0");
 }
 
 // 16 bytes per lane from global memory at base + off (base uniform, in SGPRs; off a
@@ -445,8 +450,9 @@ __device__ __forceinline__ void dma16(const void* src, lds_vptr dst) {
 __device__ __forceinline__ void dma16s(const void* base, unsigned off, lds_vptr dst) {
   const unsigned m0 = (unsigned)(__UINTPTR_TYPE__)dst;
   asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(m0), "v"(off), "s"(base)
-               : "memory");
+               : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 // The same product with the operand slabs moved HBM/L2 -> LDS by the load unit
 // (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at M0 + 16 l), three

@@ -3,8 +3,9 @@
 #include "ecg_cc.h"
 
 namespace ecg {
-// dEdh: n x n, dEdB: n x n x naux (column-major, either may be null); T, U host arrays
-// (converged amplitudes and Lambda multipliers).
+// dEdh: n x n, dEdBL / dEdBR: n x n x naux (column-major, any may be null); T, U host
+// arrays (converged amplitudes and Lambda multipliers).  Hermitian integrals: dEdBL
+// receives dE/dB and dEdBR is not written; two-sided factors: dE/dBL and dE/dBR.
 void energy_grad(Context& c, int method, const double* T1h, const double* T2h, const double* U1h, const double* U2h,
-                 double* dEdh, double* dEdB);
+                 double* dEdh, double* dEdBL, double* dEdBR);
 }  // namespace ecg

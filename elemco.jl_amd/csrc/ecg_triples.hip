@@ -42,7 +42,6 @@ struct CombArgs {
   const double* W;      // <ij|ab> as [a,b,i,j]
   const double* Y;      // T2 (or covariant U2) [a,b,i,j] for IntY
   const int* tri;       // sorted tile triples (A <= B <= C), 3 ints each
-  int boxed;            // K (and KU) stored in 8x8x8 boxes (GemmEngine boxed output), else [a,b,c]
   long kz, pz;          // batch member blockIdx.y: triplet (i + y, j, k), K/KU at + y kz, records at + y pz
   double* part;         // [ntile^3][NPART], record of output block at + nt*(bt + nt*ct)
   // per tie pattern (A==B) + 2 (B==C): the distinct output blocks, each as its
@@ -103,11 +102,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
       const int x1 = T[perm[s][0]] * TT + e % TT, x2 = T[perm[s][1]] * TT + (e / TT) % TT,
                 x3 = T[perm[s][2]] * TT + e / (TT * TT);
       const bool ok = x1 < v && x2 < v && x3 < v;
-      long off;
-      if (p.boxed)
-        off = ((long)T[perm[s][0]] + nt * ((long)T[perm[s][1]] + nt * (long)T[perm[s][2]])) * (TT * TT * TT) + e;
-      else
-        off = ok ? (long)x1 + v * x2 + v * v * x3 : 0;
+      const long off = ((long)T[perm[s][0]] + nt * ((long)T[perm[s][1]] + nt * (long)T[perm[s][2]])) * (TT * TT * TT) + e;
       box[s][e % TT + BX1 * ((e / TT) % TT) + BX2 * (e / (TT * TT))] =
           ok ? K0[off] + K1[off] + K2[off] : KT(0);
     }
@@ -161,9 +156,8 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
         xv = num / KT(eijk - ea[la] - eb[lb] - ec[lc]);
         double kk = (double)kv[0];
         if (p.KU[0]) {
-          const long off = p.boxed ? ((long)tg[0] + nt * ((long)tg[1] + nt * (long)tg[2])) * (TT * TT * TT) + la +
-                                         TT * lb + TT * TT * lc
-                                   : (long)a + v * b + v * v * c;
+          const long off = ((long)tg[0] + nt * ((long)tg[1] + nt * (long)tg[2])) * (TT * TT * TT) + la + TT * lb +
+                           TT * TT * lc;
           kk = (double)p.KU[0][off + zb * p.kz] + (double)p.KU[1][off + zb * p.kz] + (double)p.KU[2][off + zb * p.kz];
         }
         esum += kk * (double)xv;
@@ -285,59 +279,6 @@ struct TripleInts {
   const double* Wt;
   const double* Hn;
 };
-
-// K[a,b,c] for triplet (i,j,k) from amplitudes T (v,v,o,o): three segmented
-// GEMM launches, one per output orientation (cc_triples.jl:83-111 lists the
-// twelve products), each writing its own part K[0], K[1], K[2] (K is their sum;
-// no launch reads C, so no epilogue waits on memory).  Term numbers below follow
-// the order of the particle terms (1-6) and hole terms (7-12) of that loop.
-void build_K(GemmEngine& g, Prec pr, int v, int o, int i, int j, int k, const double* T, const TripleInts& I,
-             double* const K[3]) {
-  const long vv = (long)v * v, vvv = vv * v, vo = (long)v * o, vvo = vv * o;
-  auto Tp = [&](int p, int q) { return T + vv * (p + (long)o * q); };          // T[:,:,p,q]
-  auto Wm = [&](int m) { return I.W + vvv * m; };                               // W[:,:,:,m]
-  auto Wtm = [&](int m) { return I.Wt + vvv * m; };                             // Wt[:,:,:,m]
-  auto Hn = [&](int m, int n) { return I.Hn + vo * (m + (long)o * n); };      // Hn[:,:,m,n]
-  auto seg = [](GemmSeg& s, const double* A, long a_k, const double* B, long b_k, int K) {
-    s.A = A; s.a_k = a_k; s.B = B; s.b_k = b_k; s.K = K;
-  };
-  // ---- class a: K[a,(b,c)], m = a, n = (b,c) contiguous
-  {
-    GemmOp op; op.M = v; op.N = (int)vv; op.nseg = 4;
-    op.a_s0 = 1; op.b_s0 = 1;
-    seg(op.seg[0], Tp(i, j), v, Wm(k), vv, v);        // 1: T^{ij}[a,d] W_k[b,c,d]
-    seg(op.seg[1], Tp(i, k), v, Wtm(j), vv, v);       // 3: T^{ik}[a,d] W_j[c,b,d]
-    seg(op.seg[2], Hn(k, i), v, T + vv * j, vvo, o);  // 11: -H[l,a,k,i] T[b,c,j,l]
-    seg(op.seg[3], Hn(j, i), v, T + vvo * k, vv, o);  // 12: -H[l,a,j,i] T[b,c,l,k]
-    op.C = K[0]; op.c_ms0 = 1; op.c_ns0 = v;
-    op.alpha = 1.0; op.beta = 0.0;
-    g.run_prec(op, pr);
-  }
-  // ---- class b: m = (a,c) contiguous in the operands, written at a + vv*c; n = b
-  {
-    GemmOp op; op.M = (int)vv; op.N = v; op.nseg = 4;
-    op.a_s0 = 1; op.b_s0 = 1;
-    seg(op.seg[0], Wm(k), vv, Tp(j, i), v, v);        // 2: W_k[a,c,d] T^{ji}[b,d]
-    seg(op.seg[1], Wtm(i), vv, Tp(j, k), v, v);       // 5: W_i[c,a,d] T^{jk}[b,d]
-    seg(op.seg[2], T + vv * i, vvo, Hn(k, j), v, o);  // 9: T[a,c,i,l] (-H[l,b,k,j])
-    seg(op.seg[3], T + vvo * k, vv, Hn(i, j), v, o);  // 10: T[a,c,l,k] (-H[l,b,i,j])
-    op.C = K[1]; op.c_ms0 = 1; op.c_md0 = v; op.c_ms1 = vv; op.c_ns0 = v;
-    op.alpha = 1.0; op.beta = 0.0;
-    g.run_prec(op, pr);
-  }
-  // ---- class c: m = (a,b) contiguous, n = c
-  {
-    GemmOp op; op.M = (int)vv; op.N = v; op.nseg = 4;
-    op.a_s0 = 1; op.b_s0 = 1;
-    seg(op.seg[0], Wm(j), vv, Tp(k, i), v, v);        // 4: W_j[a,b,d] T^{ki}[c,d]
-    seg(op.seg[1], Wtm(i), vv, Tp(k, j), v, v);       // 6: W_i[b,a,d] T^{kj}[c,d]
-    seg(op.seg[2], T + vv * i, vvo, Hn(j, k), v, o);  // 7: T[a,b,i,l] (-H[l,c,j,k])
-    seg(op.seg[3], T + vvo * j, vv, Hn(i, k), v, o);  // 8: T[a,b,l,j] (-H[l,c,i,k])
-    op.C = K[2]; op.c_ms0 = 1; op.c_ns0 = vv;
-    op.alpha = 1.0; op.beta = 0.0;
-    g.run_prec(op, pr);
-  }
-}
 
 // Packed planes of the (T) operands (BF16 hi/lo, or one FP32 plane for FP32 products),
 // packed once per evaluation so that each triplet's three products read them directly
@@ -552,10 +493,9 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
       IL = TripleInts{vvvoL.d(), vvvoLT.d(), ovooLN.d()};
     }
   }
-  // BF16 hi/lo and FP32 products run on operands packed once for all triplets, with K in FP32;
-  // FP64 ones on FP64 planes with K in FP64 (unless the engine's FP64 plane kernel is off)
-  const bool f64p = (pr == Prec::F64 && ops.gemm.f64_kernel == 1);
-  const bool packed = (pr == Prec::BF16X3 || pr == Prec::F32 || f64p);
+  // every precision runs on operands packed once for all triplets: BF16 hi/lo and FP32
+  // planes with K in FP32, FP64 planes with K in FP64
+  const bool f64p = (pr == Prec::F64);
   const bool f32 = (pr == Prec::F32);
   const int kind = f32 ? 1 : (f64p ? 2 : 0);
   PackedInts PI, PIL;
@@ -564,32 +504,28 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   PT.T3.buf = &W[14]; PT.T4.buf = &W[15]; PT.Ts.buf = &W[16];
   PU.T3.buf = &W[17]; PU.T4.buf = &W[18]; PU.Ts.buf = &W[19];
   PIL.W.buf = &W[20]; PIL.Wt.buf = &W[21]; PIL.Hn.buf = &W[22];
-  if (packed) {
-    pack_ints(ops.gemm, v, o, IT, kind, PI);
-    pack_amps(ops.gemm, v, o, dT2.d(), kind, PT);
-    if (lambda) {
-      pack_amps(ops.gemm, v, o, dU2.d(), kind, PU);
-      if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, kind, PIL);
-    }
+  pack_ints(ops.gemm, v, o, IT, kind, PI);
+  pack_amps(ops.gemm, v, o, dT2.d(), kind, PT);
+  if (lambda) {
+    pack_amps(ops.gemm, v, o, dU2.d(), kind, PU);
+    if (!c.hermitian) pack_ints(ops.gemm, v, o, IL, kind, PIL);
   }
   const PackedInts& PIU = (lambda && !c.hermitian) ? PIL : PI;
   const int ntile = (v + TT - 1) / TT;
   const long nblocks = (long)ntile * ntile * ntile;
-  // the packed path evaluates blocks of up to BMAX triplets (i0 + z, j, k) per launch
-  const int BMAX = packed ? 8 : 1;
-  // the packed path stores K in whole 8x8x8 boxes (ntile^3 of them, padding zeroed below)
-  const long kmem = packed ? nblocks * TT * TT * TT : vvv;   // elements of one part of one triplet
-  const size_t kbytes = ((packed && !f64p) ? sizeof(float) : sizeof(double)) * kmem * BMAX;   // one part, all members
+  // blocks of up to BMAX triplets (i0 + z, j, k) per launch
+  const int BMAX = 8;
+  // K is stored in whole 8x8x8 boxes (ntile^3 of them, padding zeroed below)
+  const long kmem = nblocks * TT * TT * TT;   // elements of one part of one triplet
+  const size_t kbytes = (f64p ? sizeof(double) : sizeof(float)) * kmem * BMAX;   // one part, all members
   const long pz = nblocks * NPART;
   DevBuf& K = W[23];
   DevBuf& KU = W[24];
   DevBuf part(sizeof(double) * pz * BMAX);
   K.ensure(3 * kbytes);
   if (lambda) KU.ensure(3 * kbytes);
-  if (packed) {
-    ECG_HIP(hipMemsetAsync(K.p, 0, 3 * kbytes, s));
-    if (lambda) ECG_HIP(hipMemsetAsync(KU.p, 0, 3 * kbytes, s));
-  }
+  ECG_HIP(hipMemsetAsync(K.p, 0, 3 * kbytes, s));
+  if (lambda) ECG_HIP(hipMemsetAsync(KU.p, 0, 3 * kbytes, s));
   DevBuf IntX(sizeof(double) * n1), IntY(sizeof(double) * n1), eacc(sizeof(double) * ntile);
   ops.fill(n1, 0.0, IntX.d());
   ops.fill(n1, 0.0, IntY.d());
@@ -649,8 +585,6 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   CombArgs<float> cf{};
   cf.nv = v; cf.no = o; cf.ntile = ntile; cf.tri = ca.tri;
   cf.eo = ca.eo; cf.ev = ca.ev; cf.W = ca.W; cf.Y = ca.Y; cf.part = ca.part;
-  ca.boxed = f64p ? 1 : 0;
-  cf.boxed = 1;
   ca.kz = kmem; cf.kz = kmem;
   ca.pz = pz; cf.pz = pz;
   std::copy(&ca.nblk[0], &ca.nblk[0] + 4, &cf.nblk[0]);
@@ -663,33 +597,26 @@ void pert_triples(Context& c, bool lambda, Prec pr, const double* T1h, const dou
   // one block of triplets (i0 + z, j, k), z < nb: products, combine, and (when reduce) the
   // ordered accumulation into IntX, IntY and the energy
   auto run_block = [&](int i0, int nb, int j, int k, bool reduce) {
-    if (packed) {
-      build_K_packed(ops.gemm, v, o, i0, nb, j, k, PT, PI, Kf, Kd, kmem);
-      if (lambda) build_K_packed(ops.gemm, v, o, i0, nb, j, k, PU, PIU, KUf, KUd, kmem);
-      if (f64p) {
-        ca.i = i0; ca.j = j; ca.k = k;
-        hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, ca);
-      } else {
-        cf.i = i0; cf.j = j; cf.k = k;
-        hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, cf);
-      }
-    } else {
-      build_K(ops.gemm, pr, v, o, i0, j, k, dT2.d(), IT, Kd);
-      if (lambda) build_K(ops.gemm, pr, v, o, i0, j, k, dU2.d(), IL, KUd);
+    build_K_packed(ops.gemm, v, o, i0, nb, j, k, PT, PI, Kf, Kd, kmem);
+    if (lambda) build_K_packed(ops.gemm, v, o, i0, nb, j, k, PU, PIU, KUf, KUd, kmem);
+    if (f64p) {
       ca.i = i0; ca.j = j; ca.k = k;
-      hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid, 1u), dim3(256), 0, s, ca);
+      hipLaunchKernelGGL(triples_combine_kernel<double>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, ca);
+    } else {
+      cf.i = i0; cf.j = j; cf.k = k;
+      hipLaunchKernelGGL(triples_combine_kernel<float>, dim3(cgrid, (unsigned)nb), dim3(256), 0, s, cf);
     }
     if (reduce)
       hipLaunchKernelGGL(triples_reduce_kernel, dim3(3 * ntile), dim3(RED_T), 0, s, v, ntile, i0, nb, j, k,
                          part.d(), pz, IntX.d(), IntY.d(), eacc.d());
     ECG_HIP(hipGetLastError());
   };
-  // block size of the packed path: chosen by measurement on this context (and kept), or
-  // from the tile count of one triplet's products for short ranges
+  // block size: chosen by measurement on this context (and kept), or from the tile count
+  // of one triplet's products for short ranges
   const long ntrip_all = (long)o * (o + 1) * (o + 2) / 6 - o;
   const long nrange = count < 0 ? ntrip_all : std::min<long>(count, std::max<long>(0, ntrip_all - first));
   int nbatch = 1;
-  if (packed) {
+  {
     int& tuned = c.triples_batch[kind][lambda ? 1 : 0];
     const long tiles = (((long)v * v + 255) / 256) * ((v + 127) / 128);
     nbatch = (int)std::max<long>(1, std::min<long>(BMAX, 1024 / std::max<long>(1, tiles)));

@@ -56,10 +56,13 @@ _SIGS = {
     "ecg_lambda_solve": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp, _dp, _dp, _dp, _ip, _ip],
     "ecg_one_rdm": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_energy_grad": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_energy_grad_lr": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
     "ecg_profile": [_vp, ctypes.c_int],
     "ecg_stats": [_vp, _dp],
     "ecg_fcidump_read": [ctypes.c_char_p, ctypes.POINTER(_vp)],
     "ecg_fcidump_free": [_vp],
+    "ecg_fcidump_from_integrals": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                   ctypes.POINTER(_vp)],
     "ecg_fcidump_info": [_vp, _ip, _ip, _ip, _ip, _dp],
     "ecg_fcidump_header": [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p],
     "ecg_fcidump_header_key": [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
@@ -181,6 +184,23 @@ class Fcidump:
             raise ElemcoError(f"ecg_fcidump_read failed: {lib.ecg_last_error(None).decode()}")
         self._h, self._lib = handle, lib
         self._refresh()
+
+    @classmethod
+    def from_integrals(cls, h, eri, nelec, ms2=0, st=0, e0=0.0):
+        """In-memory dump from arrays (``ecg_fcidump_from_integrals``: the reference's ``FDump``
+        constructor, ``src/integrals/dump.jl:107-128``); ``eri[p,q,r,s] = (pq|rs)``."""
+        lib = load()
+        n = h.shape[0]
+        h = fortran(h, (n, n))
+        eri = fortran(eri, (n, n, n, n))
+        handle = _vp()
+        if lib.ecg_fcidump_from_integrals(int(n), int(nelec), int(ms2), int(st), _ptr(h), _ptr(eri), float(e0),
+                                          ctypes.byref(handle)) != 0:
+            raise ElemcoError(f"ecg_fcidump_from_integrals failed: {lib.ecg_last_error(None).decode()}")
+        self = cls.__new__(cls)
+        self._h, self._lib = handle, lib
+        self._refresh()
+        return self
 
     def _refresh(self):
         lib, handle = self._lib, self._h
@@ -441,6 +461,17 @@ class Context:
         self._chk(self._lib.ecg_energy_grad(self._h, method, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2), _ptr(gh),
                                             _ptr(gb)))
         return gh, gb
+
+    def energy_grad_lr(self, T1, T2, U1, U2, method=ECG_CCSD):
+        """``ecg_energy_grad_lr`` (two-sided factors): (dE/dh, dE/dBL, dE/dBR)."""
+        T1, T2, U1, U2 = self._t1(T1), self._t2(T2), self._t1(U1), self._t2(U2)
+        n = self.norb
+        gh = np.zeros((n, n), order="F")
+        gl = np.zeros((n, n, self.naux), order="F")
+        gr = np.zeros((n, n, self.naux), order="F")
+        self._chk(self._lib.ecg_energy_grad_lr(self._h, method, _ptr(T1), _ptr(T2), _ptr(U1), _ptr(U2), _ptr(gh),
+                                               _ptr(gl), _ptr(gr)))
+        return gh, gl, gr
 
     def profile(self, enable=True):
         """Record HIP events around every MFMA GEMM launch (``ecg_profile``)."""

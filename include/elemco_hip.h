@@ -67,6 +67,14 @@ const char* ecg_last_error(const ecg_ctx* ctx);
  * Errors: message in ecg_last_error(NULL). */
 typedef struct ecg_fcidump ecg_fcidump;
 int ecg_fcidump_read(const char* path, ecg_fcidump** fd);
+/* An in-memory dump from arrays: the FDump constructor of the reference
+ * (src/integrals/dump.jl:107-128) for four-index integrals already in memory.  h: n x n,
+ * eri: (pq|rs) at [p,q,r,s] (n^4, chemists' notation); header NORB = norb, NELEC, MS2,
+ * ISYM = 1, ORBSYM all 1, and ST when st > 0 (similarity-transformed integrals: particle
+ * symmetry only).  The arrays are copied; the dump then serves ecg_fcidump_df,
+ * ecg_fcidump_freeze, ecg_fcidump_write and ecg_create_from_fcidump_handle. */
+int ecg_fcidump_from_integrals(int norb, int nelec, int ms2, int st, const double* h, const double* eri, double e0,
+                               ecg_fcidump** fd);
 int ecg_fcidump_free(ecg_fcidump* fd);
 /* NORB, NELEC (-1 if absent), MS2 (0 if absent), ST and the core energy; any
  * output may be NULL. */
@@ -226,9 +234,19 @@ int ecg_lambda_solve(ecg_ctx* ctx, int method, int precision, const ecg_solver_o
  * U2 (ecg_lambda_solve) of the same method: dE_dh[p,q] = dE/dh[p,q] (n x n) and
  * dE_dB[p,q,Q] = dE/dB[p,q,Q] (n x n x naux), every entry of h and B treated as
  * independent; either output may be NULL.  dE/dF at fixed B equals dE/dh.
- * Hermitian integrals (BR = NULL at ecg_create) only. */
+ * Hermitian integrals (BR = NULL at ecg_create); two-sided factors: ecg_energy_grad_lr. */
 int ecg_energy_grad(ecg_ctx* ctx, int method, const double* T1, const double* T2, const double* U1,
                     const double* U2, double* dE_dh, double* dE_dB);
+/* The same for two-sided DF factors (a context made with BR, e.g. from a similarity-
+ * transformed "ST" FCIDUMP): dE_dBL and dE_dBR (n x n x naux each, either may be NULL)
+ * are the derivatives with respect to BL and BR as independent arrays.  Along directions
+ * that keep the integrals' particle symmetry (pq|rs) = (rs|pq) -- e.g. BL -> BL + X S^(1/2),
+ * BR -> BR + X S^(1/2) sign(D) for the factors BL = L |D|^(1/2), BR = L sign(D) |D|^(1/2) of
+ * ecg_fcidump_df -- their sum <dE_dBL, dBL> + <dE_dBR, dBR> is the derivative of the CC
+ * energy.  dE/dh = dE/dF at fixed factors, as for ecg_energy_grad.  Contexts with one
+ * factor (BR = NULL) are refused (use ecg_energy_grad). */
+int ecg_energy_grad_lr(ecg_ctx* ctx, int method, const double* T1, const double* T2, const double* U1,
+                       const double* U2, double* dE_dh, double* dE_dBL, double* dE_dBR);
 
 /* One-particle density: calc_1RDM (src/cc/cc_lagrange.jl:793-837); D1, dD1 n x n. */
 int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* U1, const double* U2,

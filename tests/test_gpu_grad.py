@@ -59,3 +59,47 @@ def test_energy_gradient_matches_density_and_finite_differences(method):
     fd = richardson(lambda s: solve(o, h + s * dh, B, method)[0], 1e-3)
     an = float(np.sum(gh * dh))
     assert abs(fd - an) < 1e-7 * abs(an), (fd, an)
+
+
+@pytest.mark.parametrize("method", [E.ECG_CCSD, E.ECG_DCSD])
+def test_two_sided_gradient_on_st_fixture(method):
+    """ecg_energy_grad_lr on the reference's non-Hermitian H2O_ST1 fixture (two-sided exact DF
+    factors BL = L|D|^1/2, BR = BL sign(D)): dE/dh equals dD1^T (1e-10), and along directions
+    that keep the particle symmetry of the integrals (dBL = X, dBR = X sign(D)) and along a
+    random dh, the analytic derivative equals Richardson-extrapolated central differences of
+    re-solved energies to 1e-7 relative."""
+    import os
+    from conftest import GOLDEN
+    from elemco_amd.lib import Fcidump
+    fd = Fcidump(os.path.join(GOLDEN, "H2O_ST1.FCIDUMP.gz"))
+    BL, BR = fd.df_factors()
+    h, e0, o = fd.int1.copy(order="F"), fd.int0, fd.nelec // 2
+    fd.close()
+    assert BR is not None
+    sgn = np.sign(np.einsum("pqQ,pqQ->Q", BL, BR))
+    assert np.max(np.abs(BR - BL * sgn)) < 1e-14 and (sgn < 0).any()
+
+    def solve2(hh, bl, br):
+        ctx = E.Context(o, hh.shape[0] - o, bl.shape[2], hh, bl, br, e0)
+        e, T1, T2, it, conv = ctx.cc_solve(method, E.ECG_FP64, E.default_opts(**OPTS))
+        assert conv, it
+        return float(e[0]), ctx, T1, T2
+
+    _, ctx, T1, T2 = solve2(h, BL, BR)
+    U1, U2, it, conv = ctx.lambda_solve(T1, T2, method, E.default_opts(**OPTS))
+    assert conv, it
+    with pytest.raises(E.ElemcoError, match="ecg_energy_grad_lr"):
+        ctx.energy_grad(T1, T2, U1, U2, method)
+    gh, gl, gr = ctx.energy_grad_lr(T1, T2, U1, U2, method)
+    _, dD1 = ctx.one_rdm(T1, T2, U1, U2)
+    ctx.close()
+    assert float(np.max(np.abs(gh - dD1.T))) < 1e-10 * max(1.0, float(np.max(np.abs(gh))))
+    rng = np.random.default_rng(23)
+    X = float(np.std(BL)) * rng.standard_normal(BL.shape)
+    fd_b = richardson(lambda s: solve2(h, BL + s * X, BR + s * X * sgn)[0], 1e-3)
+    an_b = float(np.sum(gl * X) + np.sum(gr * X * sgn))
+    assert abs(fd_b - an_b) < 1e-7 * abs(an_b), (fd_b, an_b)
+    dh = rng.standard_normal(h.shape)
+    fd_h = richardson(lambda s: solve2(h + s * dh, BL, BR)[0], 1e-3)
+    an_h = float(np.sum(gh * dh))
+    assert abs(fd_h - an_h) < 1e-7 * abs(an_h), (fd_h, an_h)

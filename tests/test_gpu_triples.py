@@ -115,3 +115,32 @@ def test_triples_blocks_are_tuned_and_do_not_change_results(prec, tol):
     a = EC.ctx.pert_triples_range(T1, T2, 0, 37, precision=prec)
     b = EC.ctx.pert_triples_range(T1, T2, 37, n - 37, precision=prec)
     assert abs(a[1] + b[1] - e1[1]) < 1e-12 * abs(e1[1])
+
+
+@pytest.mark.parametrize("prec,tol", [(E.ECG_FP64, 1e-10), (E.ECG_FP32, 1e-6), (E.ECG_BF16X3, 4e-5)])
+def test_non_hermitian_triples_and_lambda_triples(prec, tol):
+    """(T) and Lambda-(T) on the non-Hermitian H2O_ST1 fixture (two-sided DF factors,
+    BR != BL: the vvvoL / ovooL integrals and the PIL planes of the Lambda products).
+    Amplitudes: the oracle's MP2 guess, multipliers a seeded perturbation of it.  Bars
+    relative to the oracle's calc_pertT_closed_shell / calc_LpertT_closed_shell
+    (cc_triples.jl:39-288): 1e-10 FP64, 1e-6 FP32, 4e-5 BF16 hi/lo products."""
+    from oracle.fcidump import read_fcidump
+    path = os.path.join(GOLDEN, "H2O_ST1.FCIDUMP.gz")
+    ECo = O.ECInfo.from_fcidump(read_fcidump(path))
+    O.calc_fock_matrix(ECo)
+    O.calc_MP2(ECo)
+    T1, T2 = ECo.files["T_vo"], ECo.files["T_vvoo"]
+    rng = np.random.default_rng(17)
+    U1 = 2.0 * T1 + 0.01 * rng.standard_normal(T1.shape)
+    U2 = 2.0 * T2 - T2.transpose(0, 1, 3, 2) + 0.01 * rng.standard_normal(T2.shape)
+    U2 = 0.5 * (U2 + U2.transpose(1, 0, 3, 2))
+    ECo.files["U_vo"], ECo.files["U_vvoo"] = U1, U2
+    et_o, etb_o = OT.calc_pertT_closed_shell(ECo, T1, T2)
+    lt_o, ltb_o = OT.calc_LpertT_closed_shell(ECo)
+    EC = E.ECInfo.from_fcidump(path)
+    assert EC.ctx.naux > 0
+    et, etb = EC.ctx.pert_triples(T1, T2, precision=prec)
+    lt, ltb = EC.ctx.pert_triples(T1, T2, U1, U2, precision=prec)
+    assert lt_o != et_o
+    for got, ref in ((et, et_o), (etb, etb_o), (lt, lt_o), (ltb, ltb_o)):
+        assert abs(got - ref) < tol * abs(ref), (prec, got, ref)
