@@ -15,9 +15,14 @@
 #include "ecg_grad.h"
 #include "ecg_lambda.h"
 #include "ecg_triples.h"
+#include "ecg_ucc.h"
 
 struct ecg_ctx {
   ecg::Context c;
+};
+
+struct ecg_uctx {
+  ecg::UContext c;
 };
 
 struct ecg_fcidump {
@@ -62,6 +67,25 @@ int guarded(ecg_ctx* ctx, F&& f) {
     return -9;
   }
 }
+
+template <class F>
+int uguarded(ecg_uctx* ctx, F&& f) {
+  if (!ctx) return -1;
+  try {
+    ECG_HIP(hipSetDevice(ctx->c.device));
+    f(ctx->c);
+    ctx->c.last_error.clear();
+    return 0;
+  } catch (const ecg::Error& e) {
+    ctx->c.last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->c.last_error = e.what();
+    return -9;
+  }
+}
+
+void check_umethod(int m) { ECG_CHECK(m == ECG_UCCSD || m == ECG_UCCD, "method must be ECG_UCCSD or ECG_UCCD"); }
 
 void h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
   ECG_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
@@ -489,6 +513,67 @@ int ecg_cc_solve_batch(int nsys, ecg_ctx* const* ctxs, int method, int precision
     order[k] = k;
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  if (opts && opts->graphs && precision == ECG_FP64) {
+    // device-resident iterations: one host thread drives every system, one graph launch
+    // per system and iteration; an iteration's info is read when its event has completed,
+    // and the system's next iteration is launched at once (no barrier across systems)
+    const ecg::SolverOpts o = to_opts(opts);
+    std::vector<int> it(nsys, 0), active(nsys, 0);
+    for (int q = 0; q < nsys; ++q) {
+      const int k = order[q];
+      st[k] = guarded(ctxs[k], [&](ecg::Context& c) {
+        check_method(method);
+        c.state_init(nullptr, nullptr, o);
+        c.dev_reset(o);
+        c.dv.eager_done = false;
+        c.fp64_iterations = 0;
+        c.dev_launch(method, o);
+        ECG_HIP(hipEventRecord(c.dv.done, c.stream));
+      });
+      active[k] = st[k] == 0;
+      converged[k] = 0;
+      iterations[k] = 0;
+    }
+    int left = 0;
+    for (int k = 0; k < nsys; ++k) left += active[k];
+    while (left > 0) {
+      bool progressed = false;
+      for (int q = 0; q < nsys; ++q) {
+        const int k = order[q];
+        if (!active[k]) continue;
+        bool finished = false;
+        st[k] = guarded(ctxs[k], [&](ecg::Context& c) {
+          const hipError_t e = hipEventQuery(c.dv.done);
+          if (e == hipErrorNotReady) return;
+          ECG_HIP(e);
+          progressed = true;
+          const double* info = c.dv.info_host;
+          it[k]++;
+          c.fp64_iterations++;
+          if (info[7] != 0.0 || it[k] >= o.maxit) {
+            for (int j = 0; j < 6; ++j) energies[6 * k + j] = info[j];
+            converged[k] = info[7] != 0.0;
+            iterations[k] = it[k];
+            finished = true;
+            return;
+          }
+          c.dev_launch(method, o);
+          ECG_HIP(hipEventRecord(c.dv.done, c.stream));
+        });
+        if (finished || st[k] != 0) {
+          active[k] = 0;
+          left--;
+        }
+      }
+      if (!progressed) std::this_thread::yield();
+    }
+    int first = 0;
+    for (int k = 0; k < nsys; ++k) {
+      if (status) status[k] = st[k];
+      if (st[k] != 0 && first == 0) first = st[k];
+    }
+    return first;
+  }
   std::atomic<int> next{0};
   auto worker = [&]() {
     for (int q = next.fetch_add(1); q < nsys; q = next.fetch_add(1)) {
@@ -617,6 +702,166 @@ int ecg_stats(ecg_ctx* ctx, double* stats) {
     stats[5] = (double)c.fp64_iterations;
     stats[6] = c.ops.gemm.bytes;
     stats[7] = (double)c.triples_last_batch;
+  });
+}
+
+// ---------------------------------------------------------------- open-shell (UCCSD / UCCD)
+int ecg_ucreate(ecg_uctx** out, int nocca, int noccb, int norb, int naux, const double* h, const double* B, double e0,
+                int device, size_t workspace_bytes) {
+  if (!out) return -1;
+  *out = nullptr;
+  auto ctx = std::make_unique<ecg_uctx>();
+  ctx->c.device = device;
+  const int st = host_guarded([&] {
+    ECG_CHECK(h && B, "NULL integrals");
+    ctx->c.init(nocca, noccb, norb, naux, h, B, e0, workspace_bytes);
+    return 0;
+  });
+  if (st != 0) return st;
+  *out = ctx.release();
+  return 0;
+}
+
+int ecg_ucreate_from_fcidump(ecg_uctx** out, const char* path, int ms2, double rtol, int device,
+                             size_t workspace_bytes) {
+  if (!out || !path) return -1;
+  *out = nullptr;
+  ecg_fcidump* f = nullptr;
+  int st = ecg_fcidump_read(path, &f);
+  if (st != 0) return st;
+  std::unique_ptr<ecg_fcidump> hold(f);
+  return host_guarded([&] {
+    const long nelec = f->fd.head_int("NELEC", -1);
+    ECG_CHECK(nelec >= 0, "NELEC option not found in fcidump");
+    const long m = ms2 >= 0 ? ms2 : f->fd.head_int("MS2", 0);
+    ECG_CHECK(m >= 0 && m <= nelec && (nelec + m) % 2 == 0, "NELEC and MS2 do not give whole alpha/beta counts");
+    ECG_CHECK(!f->fd.simtra(), "the open-shell path takes Hermitian integrals (ST = 0)");
+    const ecg::DfFactors& d = fcidump_df(f, rtol);
+    ECG_CHECK(d.BR.empty(), "the open-shell path takes Hermitian integrals");
+    auto ctx = std::make_unique<ecg_uctx>();
+    ctx->c.device = device;
+    ctx->c.init((int)((nelec + m) / 2), (int)((nelec - m) / 2), f->fd.norb, d.naux, f->fd.h.data(), d.BL.data(),
+                f->fd.int0, workspace_bytes);
+    *out = ctx.release();
+    return 0;
+  });
+}
+
+int ecg_udestroy(ecg_uctx* ctx) {
+  if (!ctx) return -1;
+  (void)hipSetDevice(ctx->c.device);
+  delete ctx;
+  return 0;
+}
+
+const char* ecg_ulast_error(const ecg_uctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->c.last_error.c_str();
+}
+
+int ecg_udims(const ecg_uctx* ctx, int* norb, int* nocca, int* noccb, int* naux) {
+  if (!ctx) return -1;
+  if (norb) *norb = ctx->c.n;
+  if (nocca) *nocca = ctx->c.na;
+  if (noccb) *noccb = ctx->c.nb;
+  if (naux) *naux = ctx->c.nq;
+  return 0;
+}
+
+int ecg_ufock(ecg_uctx* ctx, double* focka, double* fockb, double* ehf) {
+  return uguarded(ctx, [&](ecg::UContext& c) {
+    const int n = c.n, N = c.N;
+    std::vector<double> f((size_t)N * N);
+    ECG_HIP(hipMemcpyAsync(f.data(), c.fock.p, sizeof(double) * f.size(), hipMemcpyDeviceToHost, c.stream));
+    ECG_HIP(hipStreamSynchronize(c.stream));
+    // spin orbital of spatial orbital p with spin s
+    auto so = [&](int p, int s) {
+      for (int k = 0; k < N; ++k)
+        if (c.sp[k] == p && c.sg[k] == s) return k;
+      return -1;
+    };
+    for (int s = 0; s < 2; ++s) {
+      double* dst = s == 0 ? focka : fockb;
+      if (!dst) continue;
+      for (int q = 0; q < n; ++q)
+        for (int p = 0; p < n; ++p) dst[p + (size_t)n * q] = f[(size_t)so(p, s) + (size_t)N * so(q, s)];
+    }
+    if (ehf) *ehf = c.ehf;
+  });
+}
+
+int ecg_ump2(ecg_uctx* ctx, double* T1a, double* T1b, double* T2a, double* T2b, double* T2ab, double* emp2) {
+  return uguarded(ctx, [&](ecg::UContext& c) {
+    ecg::DevBuf d1(sizeof(double) * c.n1()), d2(sizeof(double) * c.n2());
+    double e[1];
+    c.mp2(d1.d(), d2.d(), e);
+    std::vector<double> h1(c.n1()), h2(c.n2());
+    d2h(h1.data(), d1.p, d1.bytes, c.stream);
+    d2h(h2.data(), d2.p, d2.bytes, c.stream);
+    ECG_HIP(hipStreamSynchronize(c.stream));
+    c.to_blocks(h1.data(), h2.data(), T1a, T1b, T2a, T2b, T2ab);
+    if (emp2) *emp2 = e[0];
+  });
+}
+
+int ecg_ucc_residual(ecg_uctx* ctx, int method, const double* T1a, const double* T1b, const double* T2a,
+                     const double* T2b, const double* T2ab, double* R1a, double* R1b, double* R2a, double* R2b,
+                     double* R2ab) {
+  return uguarded(ctx, [&](ecg::UContext& c) {
+    check_umethod(method);
+    ECG_CHECK(T1a && T1b && T2a && T2b && T2ab, "NULL amplitude pointer");
+    std::vector<double> h1(c.n1()), h2(c.n2());
+    c.from_blocks(T1a, T1b, T2a, T2b, T2ab, h1.data(), h2.data());
+    if (method == ECG_UCCD) std::fill(h1.begin(), h1.end(), 0.0);
+    ecg::DevBuf d1(sizeof(double) * c.n1()), d2(sizeof(double) * c.n2()), r1(d1.bytes), r2(d2.bytes);
+    h2d(d1.p, h1.data(), d1.bytes, c.stream);
+    h2d(d2.p, h2.data(), d2.bytes, c.stream);
+    c.residual(method == ECG_UCCSD, d1.d(), d2.d(), r1.d(), r2.d());
+    d2h(h1.data(), r1.p, r1.bytes, c.stream);
+    d2h(h2.data(), r2.p, r2.bytes, c.stream);
+    ECG_HIP(hipStreamSynchronize(c.stream));
+    c.to_blocks(h1.data(), h2.data(), R1a, R1b, R2a, R2b, R2ab);
+  });
+}
+
+int ecg_ucc_solve(ecg_uctx* ctx, int method, const ecg_solver_opts* opts, double* T1a, double* T1b, double* T2a,
+                  double* T2b, double* T2ab, double* energies, int* iterations, int* converged) {
+  return uguarded(ctx, [&](ecg::UContext& c) {
+    check_umethod(method);
+    double e[4];
+    int conv = 0;
+    const int it = c.solve(method == ECG_UCCSD, to_opts(opts), e, &conv);
+    if (T1a || T1b || T2a || T2b || T2ab) {
+      std::vector<double> h1(c.n1()), h2(c.n2());
+      d2h(h1.data(), c.T1.p, sizeof(double) * c.n1(), c.stream);
+      d2h(h2.data(), c.T2.p, sizeof(double) * c.n2(), c.stream);
+      ECG_HIP(hipStreamSynchronize(c.stream));
+      c.to_blocks(h1.data(), h2.data(), T1a, T1b, T2a, T2b, T2ab);
+    }
+    if (energies) std::memcpy(energies, e, sizeof(e));
+    if (iterations) *iterations = it;
+    if (converged) *converged = conv;
+  });
+}
+
+int ecg_upert_triples(ecg_uctx* ctx, const double* T1a, const double* T1b, const double* T2a, const double* T2b,
+                      const double* T2ab, double* ET, double* ETb) {
+  return uguarded(ctx, [&](ecg::UContext& c) {
+    double e[2];
+    if (T1a || T1b || T2a || T2b || T2ab) {
+      ECG_CHECK(T1a && T1b && T2a && T2b && T2ab, "give all five amplitude blocks or none");
+      std::vector<double> h1(c.n1()), h2(c.n2());
+      c.from_blocks(T1a, T1b, T2a, T2b, T2ab, h1.data(), h2.data());
+      ecg::DevBuf d1(sizeof(double) * c.n1()), d2(sizeof(double) * c.n2());
+      h2d(d1.p, h1.data(), d1.bytes, c.stream);
+      h2d(d2.p, h2.data(), d2.bytes, c.stream);
+      c.pert_triples(d1.d(), d2.d(), e);
+    } else {
+      ECG_CHECK(c.T1.p != nullptr, "no amplitudes: run ecg_ucc_solve first or pass them");
+      c.pert_triples(c.T1.d(), c.T2.d(), e);
+    }
+    if (ET) *ET = e[0];
+    if (ETb) *ETb = e[1];
   });
 }
 

@@ -114,8 +114,38 @@ struct Context {
   std::vector<ResidualGraph> rgraphs;
   bool capturing = false;    // a residual is being recorded (no event records, no host waits)
   void residual_graph(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d);
+  // Device-resident FP64 iterations (ecg_devsolve.hip): every reduction, the DIIS solve and
+  // the convergence test on the device; the iteration is captured as one HIP graph and
+  // replayed, with one host read of its info vector per iteration.
+  struct DevIter {
+    DevBuf scal, part, tt, fvo;          // device scalars, pair-dot partials, T1 T1, f_vo
+    int maxdiis = 0;
+    bool eager_done = false;             // the first iteration after dev_reset ran eagerly
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> key;        // buffers the graph recorded
+    std::vector<double> okey;            // method and options the graph recorded
+    double* info_host = nullptr;         // pinned: info[8] of the last iteration
+    hipEvent_t done = nullptr;           // recorded after an iteration (batch driver)
+    ~DevIter();
+  } dv;
+  void dev_reset(const SolverOpts& o);
+  void dev_step(int method, const SolverOpts& o);
+  void dev_launch(int method, const SolverOpts& o);
+  int dev_solve(int method, const SolverOpts& o, double* energies, int* converged, FILE* log);
   ~Context();
 };
+
+// Half of the DF particle-particle ladder: R2r[a,b,i,j] += sum_Q sum_cd Lf[a,c,Q] Rf[b,d,Q]
+// X[c,d,i,j] for b >= the start of a's block of a-rows (diagonal blocks weighted 1/2), X given
+// as Xx[i,j,c,d]; when the full term P satisfies P[a,b,i,j] = P[b,a,j,i] it is completed by
+// R2 += R2r + R2r[b,a,j,i].  The integrals V[a',b',c,d] of a block of a-rows are assembled on
+// MFMA in the free part of the contraction arena; no v^4 array is formed.
+void df_ladder_half(TensorOps& ops, long nv, long no, int nq, Prec pr, const double* Lf, const double* Rf,
+                    const double* T2x, double* R2r);
+
+// Symmetric eigen-decomposition of a small m x m row-major matrix A (destroyed) by cyclic
+// Jacobi sweeps: eigenvectors in the columns of V, eigenvalues w (the DIIS solves).
+void jacobi_eig(int m, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w);
 
 // Per-(i,j) block sums of two [a,b,i,j] tensors (v = nv, npairs = o^2 blocks):
 // part[2 p] = sum_ab X[a,b,p] Y[a,b,p],  part[2 p + 1] = sum_ab X[b,a,p] Y[a,b,p].

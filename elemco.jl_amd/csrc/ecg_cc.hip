@@ -311,13 +311,17 @@ void Context::ladder(Prec pr, const double* T2x, double* R2r) {
 // out[a,b,i,j] += sum_Q sum_cd Lf[a,c,Q] Rf[b,d,Q] X[c,d,i,j] for b >= (start of a's block),
 // diagonal blocks weighted 1/2 (see ladder); X is supplied as Xx[i,j,c,d].
 void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const double* T2x, double* R2r) {
+  df_ladder_half(ops, nv, no, nq, pr, Lf, Rf, T2x, R2r);
+}
+
+void df_ladder_half(TensorOps& ops, long nv, long no, int nq, Prec pr, const double* Lf, const double* Rf,
+                    const double* T2x, double* R2r) {
   size_t mark = ops.arena.mark();
   size_t free_b = ops.arena.buf.bytes - ops.arena.mark();
   const long v = nv;
   const long per_a = v * v * v;
   long ab = std::max<long>(1, std::min<long>(v, (long)(free_b * 0.8 / sizeof(double)) / per_a));
   double* V = ops.arena.take(ab * per_a);
-  ops.gemm.stream = stream;
   // T2x is read-only here: its packed BF16 planes serve every block's second product
   ops.gemm.drop_kept_planes();
   for (long a0 = 0; a0 < v; a0 += ab) {
@@ -339,9 +343,9 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
       const long nbp = part == 0 ? na : nb - na;
       if (nbp <= 0) continue;
       GemmOp h;
-      h.M = (int)(na * nbp); h.N = no * no; h.K = (int)(v * v);
+      h.M = (int)(na * nbp); h.N = (int)(no * no); h.K = (int)(v * v);
       h.A = V + na * b0; h.a_s0 = 1; h.a_d0 = (int)na; h.a_s1 = na; h.a_k = na * nb;
-      h.B = T2x; h.b_s0 = 1; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = (long)no * no;
+      h.B = T2x; h.b_s0 = 1; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = no * no;
       h.C = R2r + a0 + (a0 + b0) * v; h.c_ms0 = 1; h.c_md0 = (int)na; h.c_ms1 = v;
       h.c_ns0 = v * v; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
       h.alpha = part == 0 ? 0.5 : 1.0; h.beta = 1.0;
@@ -667,7 +671,7 @@ void Context::diis_reset(int maxd, double resthr) {
 }
 
 // symmetric eigen-decomposition by cyclic Jacobi (small matrices)
-static void jacobi_eig(int m, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w) {
+void jacobi_eig(int m, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w) {
   V.assign((size_t)m * m, 0.0);
   for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
   for (int sweep = 0; sweep < 100; ++sweep) {
@@ -858,6 +862,8 @@ void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info,
 // plain FP64 ones with a fresh DIIS history (needed only for thresholds below
 // that floor).
 int Context::solve(int method, Prec pr, const SolverOpts& o, double* energies, int* converged, FILE* log) {
+  // FP64 with graphs: whole iterations on the device, replayed from one captured graph
+  if (pr == Prec::F64 && o.graphs) return dev_solve(method, o, energies, converged, log);
   diis_reset(o.maxdiis, o.resthr);
   iter = 0;
   have_delta = false;

@@ -101,7 +101,7 @@ void energy_grad(Context& c, int method, const double* T1h, const double* T2h, c
   Tensor goo = V3(Goo.d(), o, o, Q), gov = V3(Gov.d(), o, v, Q), gvo = V3(Gvo.d(), v, o, Q),
          gvv = V3(Gvv.d(), v, v, Q);
   Tensor groo = herm ? goo : V3(GRoo.d(), o, o, Q), grov = herm ? gov : V3(GRov.d(), o, v, Q),
-         grvo = herm ? gvo : V3(GRvo.d(), v, o, Q), grvv = herm ? gvv : V3(GRvv.d(), v, v, Q);
+         grvo = herm ? gvo : V3(GRvo.d(), v, o, Q);   // the right vv adjoint is written by the ladder only
   Tensor loo = V3(D.oo.d(), o, o, Q), lov = V3(D.ov.d(), o, v, Q), lvo = V3(D.vo.d(), v, o, Q),
          lvv = V3(D.vv.d(), v, v, Q);
   Tensor roo = V3(DR.oo.d(), o, o, Q), rov = V3(DR.ov.d(), o, v, Q), rvo = V3(DR.vo.d(), v, o, Q),

@@ -193,3 +193,38 @@ def ccdriver(EC: ECInfo, method: str):
         out[name + "(T)c"] = ecc["E"] + t["ET3"]
         out[name + "(T)"] = ecc["E"] + t["ET3"] + ehf
     return out
+
+
+def ccdriver_unrestricted(fcidump: str, method: str, ms2=None, thr=1e-10, maxit=100):
+    """``ccdriver(EC, method; fcidump)`` for the open-shell methods ``uccsd``, ``uccd``, ``uccsd(t)``
+    (``src/cc/ccdriver.jl:37-76,348-393``; ``@opt wf ms2=...`` is ``ms2``, None keeps the file's MS2).
+
+    Returns the reference's energy keys: ``HF``, ``MP2``, ``MP2c``, ``<M>``, ``<M>c`` and for
+    ``uccsd(t)`` ``[T]``, ``(T)``, ``UCCSD(T)``, ``UCCSD(T)c``.
+    """
+    m = method.lower()
+    pert = m.endswith("(t)")
+    base = m[:-3] if pert else m
+    if base not in ("uccsd", "uccd") or (pert and base != "uccsd"):
+        raise lib.ElemcoError(f"method {method} is not on the open-shell path (uccsd, uccd, uccsd(t))")
+    ctx = lib.UContext.from_fcidump(fcidump, -1 if ms2 is None else int(ms2))
+    try:
+        out = {}
+        _, _, ehf = ctx.fock()
+        out["HF"] = ehf
+        _, emp2 = ctx.mp2()
+        out["MP2c"], out["MP2"] = emp2, emp2 + ehf
+        e, _, it, conv = ctx.solve(lib.ECG_UCCSD if base == "uccsd" else lib.ECG_UCCD,
+                                   lib.default_opts(thr=thr, maxit=maxit))
+        if not conv:
+            print("WARNING: CC iterations did not converge!")
+        name = base.upper()
+        out[name + "c"], out[name] = e[0], e[0] + ehf
+        out["iterations"], out["converged"] = it, conv
+        if pert:
+            et, etb = ctx.pert_triples()
+            out["[T]"], out["(T)"] = etb, et
+            out["UCCSD(T)c"], out["UCCSD(T)"] = e[0] + et, e[0] + et + ehf
+        return out
+    finally:
+        ctx.close()

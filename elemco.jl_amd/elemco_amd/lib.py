@@ -81,7 +81,22 @@ _SIGS = {
     "ecg_create_from_ao": [ctypes.POINTER(_vp), ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
                            ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_size_t],
 }
-_RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p}
+_SIGS.update({
+    "ecg_ucreate": [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                    ctypes.c_double, ctypes.c_int, ctypes.c_size_t],
+    "ecg_ucreate_from_fcidump": [ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_size_t],
+    "ecg_udestroy": [_vp],
+    "ecg_ulast_error": [_vp],
+    "ecg_udims": [_vp, _ip, _ip, _ip, _ip],
+    "ecg_ufock": [_vp, _dp, _dp, _dp],
+    "ecg_ump2": [_vp, _dp, _dp, _dp, _dp, _dp, _dp],
+    "ecg_ucc_residual": [_vp, ctypes.c_int] + [_dp] * 10,
+    "ecg_ucc_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts)] + [_dp] * 6 + [_ip, _ip],
+    "ecg_upert_triples": [_vp] + [_dp] * 7,
+})
+ECG_UCCSD, ECG_UCCD = 0, 1
+_RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p, "ecg_ulast_error": ctypes.c_char_p}
 EXPORTED = tuple(_SIGS)
 
 _lib = None
@@ -482,6 +497,100 @@ class Context:
         self._chk(self._lib.ecg_stats(self._h, _ptr(s)))
         return dict(residual_ms=s[0], ladder_ms=s[1], gemm_ms=s[2], gemm_launches=int(s[3]), gemm_flops=s[4],
                     fp64_iterations=int(s[5]), gemm_bytes=s[6], triples_block=int(s[7]))
+
+
+class UContext:
+    """Open-shell context (``ecg_ucreate``): UCCSD / UCCD on one GPU.  Amplitudes in the
+    reference's spin blocks (T1a, T1b, T2a, T2b, T2ab)."""
+
+    def __init__(self, nocca, noccb, h, B, e0=0.0, device=0, workspace_bytes=0):
+        lib = load()
+        n, _, naux = B.shape
+        h = fortran(h, (n, n))
+        B = fortran(B, (n, n, naux))
+        handle = _vp()
+        st = lib.ecg_ucreate(ctypes.byref(handle), int(nocca), int(noccb), int(n), int(naux), _ptr(h), _ptr(B),
+                             float(e0), int(device), int(workspace_bytes))
+        if st != 0:
+            raise ElemcoError(f"ecg_ucreate failed ({st}): {lib.ecg_ulast_error(None).decode()}")
+        self._h, self._lib = handle, lib
+        self._dims()
+
+    @classmethod
+    def from_fcidump(cls, path, ms2=-1, rtol=0.0, device=0, workspace_bytes=0):
+        """``ecg_ucreate_from_fcidump``: nocca/noccb from NELEC and ms2 (the header's MS2 when ms2 < 0)."""
+        lib = load()
+        handle = _vp()
+        st = lib.ecg_ucreate_from_fcidump(ctypes.byref(handle), os.fsencode(path), int(ms2), float(rtol), int(device),
+                                          int(workspace_bytes))
+        if st != 0:
+            raise ElemcoError(f"ecg_ucreate_from_fcidump failed ({st}): {lib.ecg_ulast_error(None).decode()}")
+        self = cls.__new__(cls)
+        self._h, self._lib = handle, lib
+        self._dims()
+        return self
+
+    def _dims(self):
+        n, a, b, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.ecg_udims(self._h, ctypes.byref(n), ctypes.byref(a), ctypes.byref(b), ctypes.byref(q)))
+        self.norb, self.nocca, self.noccb, self.naux = n.value, a.value, b.value, q.value
+        self.nvirta, self.nvirtb = self.norb - self.nocca, self.norb - self.noccb
+
+    def _chk(self, st):
+        if st != 0:
+            raise ElemcoError(self._lib.ecg_ulast_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ecg_udestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def zeros(self):
+        na, nb, va, vb = self.nocca, self.noccb, self.nvirta, self.nvirtb
+        return [np.zeros(s, order="F") for s in ((va, na), (vb, nb), (va, va, na, na), (vb, vb, nb, nb),
+                                                 (va, vb, na, nb))]
+
+    def fock(self):
+        n = self.norb
+        fa, fb = np.zeros((n, n), order="F"), np.zeros((n, n), order="F")
+        e = ctypes.c_double()
+        self._chk(self._lib.ecg_ufock(self._h, _ptr(fa), _ptr(fb), ctypes.byref(e)))
+        return fa, fb, e.value
+
+    def mp2(self):
+        T = self.zeros()
+        e = ctypes.c_double()
+        self._chk(self._lib.ecg_ump2(self._h, *[_ptr(t) for t in T], ctypes.byref(e)))
+        return T, e.value
+
+    def residual(self, T, method=ECG_UCCSD):
+        T = [fortran(t, z.shape) for t, z in zip(T, self.zeros())]
+        R = self.zeros()
+        self._chk(self._lib.ecg_ucc_residual(self._h, int(method), *[_ptr(t) for t in T], *[_ptr(r) for r in R]))
+        return R
+
+    def solve(self, method=ECG_UCCSD, opts=None):
+        """Returns (energies [E_H, E_proj, dE, NormR], amplitudes, iterations, converged)."""
+        opts = opts or default_opts()
+        T = self.zeros()
+        e = np.zeros(4)
+        it, conv = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.ecg_ucc_solve(self._h, int(method), ctypes.byref(opts), *[_ptr(t) for t in T], _ptr(e),
+                                          ctypes.byref(it), ctypes.byref(conv)))
+        return e, T, it.value, bool(conv.value)
+
+    def pert_triples(self, T=None):
+        """``ecg_upert_triples``: (E(T), E[T]) at the blocks T (or the last solve's amplitudes)."""
+        et, etb = ctypes.c_double(), ctypes.c_double()
+        ptrs = [None] * 5 if T is None else [_ptr(fortran(t, z.shape)) for t, z in zip(T, self.zeros())]
+        self._chk(self._lib.ecg_upert_triples(self._h, *ptrs, ctypes.byref(et), ctypes.byref(etb)))
+        return et.value, etb.value
 
 
 def cc_solve_batch(contexts, method=ECG_CCSD, precision=ECG_FP64, opts=None, max_threads=0):

@@ -265,6 +265,51 @@ int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* 
 int ecg_profile(ecg_ctx* ctx, int enable);
 int ecg_stats(ecg_ctx* ctx, double* stats);
 
+/* ---- Open-shell (unrestricted) coupled cluster: UCCSD (ECG_UCCSD) and UCCD (ECG_UCCD, no
+ * singles).  Replaces the unrestricted branch of calc_cc (src/cc/cc.jl:2451-2516) with its
+ * residual calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab; dc=false) (src/cc/cc.jl:1496-1934),
+ * calc_UMP2 (cc.jl:1034-1060) and the open-shell Fock / HF energy (calc_fock_matrix,
+ * calc_HF_energy, src/cc/cctools.jl:35-76).  The alpha spin orbitals are the lowest nocca
+ * orbitals, the beta ones the lowest noccb (the default spaces o / O of setup_space_fd!,
+ * src/infos/ecinfos.jl:171-192).  Amplitudes and residuals in the reference's spin blocks,
+ * column-major: T1a [va, na], T1b [vb, nb], T2a [va, va, na, na], T2b [vb, vb, nb, nb] (both
+ * antisymmetric), T2ab [va, vb, na, nb], with na = nocca, nb = noccb, va = norb - na,
+ * vb = norb - nb.  Hermitian integrals, (pq|rs) = sum_Q B[p,q,Q] B[r,s,Q].  Status codes and
+ * errors as for the closed-shell context (message: ecg_ulast_error). */
+enum { ECG_UCCSD = 0, ECG_UCCD = 1 };
+typedef struct ecg_uctx ecg_uctx;
+int ecg_ucreate(ecg_uctx** ctx, int nocca, int noccb, int norb, int naux, const double* h, const double* B,
+                double e0, int device, size_t workspace_bytes);
+/* From an FCIDUMP file (ST = 0): nocca = (NELEC + ms2) / 2, noccb = (NELEC - ms2) / 2, with ms2
+ * the header's MS2 when ms2 < 0 (the reference's `@opt wf ms2=...` overrides it). */
+int ecg_ucreate_from_fcidump(ecg_uctx** ctx, const char* path, int ms2, double rtol, int device,
+                             size_t workspace_bytes);
+int ecg_udestroy(ecg_uctx* ctx);
+/* Message of the last failed call on ctx; with ctx = NULL, of the last failed create call. */
+const char* ecg_ulast_error(const ecg_uctx* ctx);
+int ecg_udims(const ecg_uctx* ctx, int* norb, int* nocca, int* noccb, int* naux);
+/* alpha and beta Fock matrices (gen_fock(EC, :alpha / :beta), n x n each) and the HF energy
+ * 0.5 (sum eps_o^a + sum eps_o^b + sum h_oo^a + sum h_oo^b) + E0; any output may be NULL. */
+int ecg_ufock(ecg_uctx* ctx, double* focka, double* fockb, double* ehf);
+/* UMP2 amplitudes and energy (doubles plus the Fock part of the singles energy). */
+int ecg_ump2(ecg_uctx* ctx, double* T1a, double* T1b, double* T2a, double* T2b, double* T2ab, double* emp2);
+/* The residual of the given amplitudes (UCCD: the singles are taken as zero and R1 is zero). */
+int ecg_ucc_residual(ecg_uctx* ctx, int method, const double* T1a, const double* T1b, const double* T2a,
+                     const double* T2b, const double* T2ab, double* R1a, double* R1b, double* R2a, double* R2b,
+                     double* R2ab);
+/* cc_iterations! (src/cc/cc.jl:2518-2648) from the UMP2 guess with the reference's
+ * unrestricted DIIS dots; energies = [E_H, E_proj, dE, NormR] of the last iteration, the
+ * converged amplitudes in T1a .. T2ab when given (any may be NULL).  Uses opts->thr, conven,
+ * maxit, shifts, shiftp, maxdiis, resthr (FP64 throughout). */
+int ecg_ucc_solve(ecg_uctx* ctx, int method, const ecg_solver_opts* opts, double* T1a, double* T1b, double* T2a,
+                  double* T2b, double* T2ab, double* energies, int* iterations, int* converged);
+/* (T) of UCCSD: calc_pertT_unrestricted (src/cc/cc_triples.jl:297-320, same-spin and mixed-spin
+ * parts :357-640) at the given amplitudes, or at those of the last ecg_ucc_solve when all five
+ * are NULL; E(T) and E[T].  Diagonal-Fock denominators and the f_ov term of the reference.  Holds
+ * five (nvirta + nvirtb)^3 (nocca + noccb)^3 arrays in the workspace. */
+int ecg_upert_triples(ecg_uctx* ctx, const double* T1a, const double* T1b, const double* T2a, const double* T2b,
+                      const double* T2ab, double* ET, double* ETb);
+
 #ifdef __cplusplus
 }
 #endif

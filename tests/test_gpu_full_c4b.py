@@ -6,6 +6,10 @@ Bars:
   own single-system ecg_cc_solve to 1e-10 Eh;
 * the batch with FP32 products (defect-corrected, ending on FP64 iterations)
   is within 1e-9 Eh of the FP64 batch for every system;
+* the batch of device-resident iterations (opts.graphs = 1: reductions, DIIS and the
+  convergence test on the device, one captured graph per system replayed every
+  iteration, one host thread driving all systems) takes the same iterations as the
+  host-driven batch and gives its energies to 1e-10 Eh;
 * the smallest and the largest system match the CPU oracle's calc_cc (the
   reference's cc_iterations!, cc.jl:2518-2648) to 1e-9 Eh.
 """
@@ -53,6 +57,12 @@ def test_c4b_batch_matches_single_solves(batch):
     d = float(np.max(np.abs(e32[:, 0] - e64[:, 0])))
     print(f"  max |E_mixed - E_fp64| = {d:.2e}", flush=True)
     assert d < 1e-9
+    E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, E.default_opts(maxit=100, thr=THR, graphs=1))   # captures
+    eg, itg, convg = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, E.default_opts(maxit=100, thr=THR, graphs=1))
+    assert convg.all()
+    dg = float(np.max(np.abs(eg[:, 0] - e64[:, 0])))
+    print(f"  device-resident batch: max |E - E_host| = {dg:.2e}, iterations equal: {(itg == it64).all()}", flush=True)
+    assert dg < 1e-10 and (itg == it64).all()
     # the extreme systems against the oracle
     for k in (int(np.argmin([s[0] for s in sizes])), int(np.argmax([s[0] for s in sizes]))):
         o, v, q, seed, sc = sizes[k]

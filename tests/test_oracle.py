@@ -295,3 +295,100 @@ def test_write_fcidump_oracle_roundtrip(tmp_path, name):
         keep1 = np.where(np.abs(fd.int1) > tol, fd.int1, 0.0)
         assert np.max(np.abs(back.int2 - keep2)) < 1e-15 and np.max(np.abs(back.int1 - keep1)) < 1e-15
         assert back.int0 == fd.int0
+
+
+# ---------------------------------------------------------------- open-shell (UCCSD) oracle
+def _uccsd_system(name, ms2):
+    from oracle import uccsd as U
+    fd = read_fcidump(os.path.join(GOLDEN, name))
+    na, nb = (fd.nelec + ms2) // 2, (fd.nelec - ms2) // 2
+    return U, U.USystem(fd.int1, fd.int2.transpose(0, 2, 1, 3), fd.int0, na, nb), fd
+
+
+def test_uccsd_closed_shell_limit_equals_ccsd_residual():
+    """With nocca = noccb the spin-orbital UCCSD residual's blocks are the closed-shell residual
+    (calc_cc_resid, cc.jl:1343-1489): R1a = R1b = R1, R2ab = R2, R2a = R2b = R2 - R2[b,a,i,j], for
+    random amplitudes (every term of both residuals contributes)."""
+    U, S, fd = _uccsd_system("H2O.FCIDUMP.gz", 0)
+    EC = ECInfo.from_fcidump(fd)
+    calc_fock_matrix(EC)
+    assert abs(U.hf_energy(S) - calc_HF_energy(EC)) < 1e-12
+    rng = np.random.default_rng(3)
+    no, nv = EC.nocc, EC.nvirt
+    T1 = 0.05 * rng.standard_normal((nv, no))
+    T2 = 0.05 * rng.standard_normal((nv, nv, no, no))
+    T2 = T2 + T2.transpose(1, 0, 3, 2)
+    R1c, R2c = calc_cc_resid(EC, T1, T2)
+    Ts = S.from_blocks(T1, T1, T2 - T2.transpose(1, 0, 2, 3), T2 - T2.transpose(1, 0, 2, 3), T2)
+    R1a, R1b, R2a, R2b, R2ab = S.to_blocks(*U.calc_ucc_resid(S, *Ts))
+    R2s = R2c - R2c.transpose(1, 0, 2, 3)
+    for got, ref in ((R1a, R1c), (R1b, R1c), (R2ab, R2c), (R2a, R2s), (R2b, R2s)):
+        assert np.max(np.abs(got - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("name,ms2,method,ref", [
+    ("H2O_CATION.FCIDUMP.gz", 1, "uccsd", {"HF": -75.337282954481, "MP2c": -0.207727619864, "UCCSDc": -0.232320803220}),
+    ("H2O_CATION.FCIDUMP.gz", 1, "uccd", {"UCCDc": -0.213495204514}),
+    ("H2O.FCIDUMP.gz", 2, "uccsd", {"HF": -75.62407982361415, "MP2c": -0.22401008330020, "UCCSDc": -0.276560985613}),
+])
+def test_uccsd_reference_energies(name, ms2, method, ref):
+    """test/h2o_cation.jl:4-26 (H2O+ doublet on ROHF orbitals: HF, MP2, UCCSD, UCCD) and
+    test/h2o_triplet.jl:4-21 (H2O triplet, ms2 = 2: HF, MP2, UCCSD), at the reference's 1e-6."""
+    from oracle import uccsd as U
+    fd = read_fcidump(os.path.join(GOLDEN, name))
+    na, nb = (fd.nelec + ms2) // 2, (fd.nelec - ms2) // 2
+    e = U.ucc_driver(fd.int1, fd.int2.transpose(0, 2, 1, 3), fd.int0, na, nb, thr=1e-14, method=method)
+    assert e["converged"]
+    for k, v in ref.items():
+        assert abs(e[k] - v) < EPS_REF, (k, e[k], v)
+
+
+def test_uccsd_two_same_spin_electrons_is_fci():
+    """Two alpha electrons (Sz = 1): UCCSD is exact, so E_corr = E_FCI - E_HF with E_FCI the lowest
+    eigenvalue of H in the two-electron determinant space |pq>, p < q (Slater-Condon rules:
+    H_(pq),(rs) = h_pr d_qs - h_ps d_qr - h_qr d_ps + h_qs d_pr + <pq||rs>)."""
+    from oracle import uccsd as U
+    from oracle.synthetic import make_df_system
+    s = make_df_system(2, 6, 20, 17, scale=2.0)
+    n = s.h.shape[0]
+    eri = np.einsum('pqQ,rsQ->pqrs', s.B, s.B)
+    S = U.USystem(s.h, eri, s.e0, 2, 0)
+    E, _, _, conv, _ = U.calc_ucc(S, thr=1e-20, maxit=300)
+    assert conv
+    pairs = [(p, q) for p in range(n) for q in range(p + 1, n)]
+    h = s.h
+    A = lambda p, q, r, t: eri[p, r, q, t] - eri[p, t, q, r]
+    d = lambda a, b: 1.0 if a == b else 0.0
+    H = np.array([[h[p, r] * d(q, t) - h[p, t] * d(q, r) - h[q, r] * d(p, t) + h[q, t] * d(p, r) + A(p, q, r, t)
+                   for (r, t) in pairs] for (p, q) in pairs])
+    efci = float(np.linalg.eigvalsh(H)[0]) + s.e0
+    ehf = H[0, 0] + s.e0
+    assert abs(U.hf_energy(S) - ehf) < 1e-12
+    assert abs(E - (efci - ehf)) < 1e-10
+
+
+def test_unrestricted_triples_closed_shell_limit_and_reference():
+    """(T) of UCCSD in spin-orbital form (oracle/uccsd.py, cc_triples.jl:297-640): with nocca = noccb
+    it equals the closed-shell (T) oracle (calc_pertT_closed_shell, cc_triples.jl:39-148) for
+    perturbed MP2 amplitudes (1e-14), and on the H2O triplet it gives the reference's
+    UCCSD(T) energy of test/h2o_triplet.jl:8 (1e-6)."""
+    from oracle import uccsd as U
+    fd = read_fcidump(H2O)
+    eri = fd.int2.transpose(0, 2, 1, 3)
+    no = fd.nelec // 2
+    S = U.USystem(fd.int1, eri, fd.int0, no, no)
+    EC = ECInfo.from_fcidump(fd)
+    calc_fock_matrix(EC)
+    from oracle.ccsd import calc_MP2
+    calc_MP2(EC)
+    T1 = EC.files["T_vo"] + 0.01 * np.random.default_rng(2).standard_normal(EC.files["T_vo"].shape)
+    T2 = EC.files["T_vvoo"]
+    et, etb = triples.calc_pertT_closed_shell(EC, T1, T2)
+    Ts = S.from_blocks(T1, T1, T2 - T2.transpose(1, 0, 2, 3), T2 - T2.transpose(1, 0, 2, 3), T2)
+    eu, ebu = U.calc_pertT_unrestricted(S, *Ts)
+    assert abs(eu - et) < 1e-14 and abs(ebu - etb) < 1e-14
+    St = U.USystem(fd.int1, eri, fd.int0, (fd.nelec + 2) // 2, (fd.nelec - 2) // 2)
+    E, T1t, T2t, conv, _ = U.calc_ucc(St, thr=1e-14)
+    assert conv
+    e3, _ = U.calc_pertT_unrestricted(St, T1t, T2t)
+    assert abs(E + e3 - (-0.295388574918)) < EPS_REF
