@@ -85,7 +85,9 @@ int uguarded(ecg_uctx* ctx, F&& f) {
   }
 }
 
-void check_umethod(int m) { ECG_CHECK(m == ECG_UCCSD || m == ECG_UCCD, "method must be ECG_UCCSD or ECG_UCCD"); }
+void check_umethod(int m) {
+  ECG_CHECK(m == ECG_UCCSD || m == ECG_UCCD || m == ECG_UDCSD, "method must be ECG_UCCSD, ECG_UCCD or ECG_UDCSD");
+}
 
 void h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
   ECG_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
@@ -705,7 +707,7 @@ int ecg_stats(ecg_ctx* ctx, double* stats) {
   });
 }
 
-// ---------------------------------------------------------------- open-shell (UCCSD / UCCD)
+// ---------------------------------------------------------------- open-shell (UCCSD / UCCD / UDCSD)
 int ecg_ucreate(ecg_uctx** out, int nocca, int noccb, int norb, int naux, const double* h, const double* B, double e0,
                 int device, size_t workspace_bytes) {
   if (!out) return -1;
@@ -816,7 +818,7 @@ int ecg_ucc_residual(ecg_uctx* ctx, int method, const double* T1a, const double*
     ecg::DevBuf d1(sizeof(double) * c.n1()), d2(sizeof(double) * c.n2()), r1(d1.bytes), r2(d2.bytes);
     h2d(d1.p, h1.data(), d1.bytes, c.stream);
     h2d(d2.p, h2.data(), d2.bytes, c.stream);
-    c.residual(method == ECG_UCCSD, d1.d(), d2.d(), r1.d(), r2.d());
+    c.residual(method, d1.d(), d2.d(), r1.d(), r2.d());
     d2h(h1.data(), r1.p, r1.bytes, c.stream);
     d2h(h2.data(), r2.p, r2.bytes, c.stream);
     ECG_HIP(hipStreamSynchronize(c.stream));
@@ -830,7 +832,7 @@ int ecg_ucc_solve(ecg_uctx* ctx, int method, const ecg_solver_opts* opts, double
     check_umethod(method);
     double e[4];
     int conv = 0;
-    const int it = c.solve(method == ECG_UCCSD, to_opts(opts), e, &conv);
+    const int it = c.solve(method, to_opts(opts), e, &conv);
     if (T1a || T1b || T2a || T2b || T2ab) {
       std::vector<double> h1(c.n1()), h2(c.n2());
       d2h(h1.data(), c.T1.p, sizeof(double) * c.n1(), c.stream);

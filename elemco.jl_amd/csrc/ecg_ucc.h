@@ -34,12 +34,14 @@ struct UContext {
   long n2() const { return (long)V * V * O * O; }
   void init(int nocca, int noccb, int norb, int naux, const double* h, const double* B, double e0_in,
             size_t workspace_bytes);
-  // residual R(T) of UCCSD (singles = true) or UCCD (T1 = 0, R1 set to zero); device pointers
-  void residual(bool singles, const double* T1d, const double* T2d, double* R1d, double* R2d);
+  // method: 0 UCCSD, 1 UCCD (T1 = 0, R1 set to zero), 2 UDCSD (distinguishable-cluster doubles)
+  static bool has_singles(int method) { return method != 1; }
+  // residual R(T) of the method at device amplitudes T1d, T2d
+  void residual(int method, const double* T1d, const double* T2d, double* R1d, double* R2d);
   void mp2(double* T1d, double* T2d, double* emp2);
   double proj_energy(const double* T1d, const double* T2d);
   double dot(const double* a, const double* b, long n) { return ops.dot_host(n, a, b); }
-  int solve(bool singles, const SolverOpts& o, double* energies, int* converged);
+  int solve(int method, const SolverOpts& o, double* energies, int* converged);
   // (T) of UCCSD on spin orbitals at amplitudes T1d, T2d (device): e = [E(T), E[T]]
   void pert_triples(const double* T1d, const double* T2d, double* e);
   // spin blocks of the reference (T1a, T1b, T2a, T2b, T2ab) <-> spin-orbital arrays (host)

@@ -1,7 +1,8 @@
-// Open-shell (unrestricted) CCSD / CCD on the device.
+// Open-shell (unrestricted) CCSD / CCD / DCSD on the device.
 //
-// The reference's UCCSD residual (calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab), src/cc/cc.jl:
-// 1496-1934) works on alpha/beta spin blocks with dressed four-index integral files.  Here the
+// The reference's UCCSD / UDCSD residual (calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab; dc),
+// src/cc/cc.jl:1496-1934) works on alpha/beta spin blocks with dressed four-index integral
+// files.  Here the
 // same equations run on spin orbitals ([occupied alpha, occupied beta, virtual alpha, virtual
 // beta]) in the intermediate form of Stanton and Gauss (J. Chem. Phys. 94, 4334 (1991)) with
 // the full Fock matrix (non-canonical ROHF orbitals), as a sequence of contractions on the
@@ -11,6 +12,14 @@
 // particle-particle ladder sum_ef <ab|ef> tau_ij^ef is assembled from the factors block by
 // block (df_ladder_half), so no v^4 array exists.  The alpha-alpha, beta-beta and alpha-beta
 // blocks of the residual are the reference's R2a, R2b, R2ab.
+//
+// UDCSD: the reference's dressed equations differ from UCCSD only in terms quadratic in T2
+// with the undressed <kl|cd> (cc.jl:1724-1808): the hole-hole term is dropped, the quadratic
+// parts of x_ij / x_ab are halved and the ring terms keep only <kl|cd> with (k,c) in one
+// amplitude and (l,d) in the other.  Here those differences are taken out of the UCCSD terms
+// (with T2 alone, the T1 parts of tau / tau~ untouched): W_mbej gets 1/2 sum_nf <mn|fe>
+// T2[b,f,j,n] back, the doubles' copies of F_be and F_mj a quarter of their T2 parts, and
+// 1/4 T2[a,b,m,n] sum_ef <mn||ef> T2[e,f,i,j] is subtracted.
 #include "ecg_ucc.h"
 
 #include <algorithm>
@@ -173,7 +182,8 @@ UContext::~UContext() {
 }
 
 // R(T) on spin orbitals: Stanton-Gauss intermediates with the full Fock matrix.
-void UContext::residual(bool singles, const double* T1d, const double* T2d, double* R1d, double* R2d) {
+void UContext::residual(int method, const double* T1d, const double* T2d, double* R1d, double* R2d) {
+  const bool singles = has_singles(method), dc = method == 2;
   const long o = O, v = V;
   const Prec F = Prec::F64;
   ops.arena.reset();
@@ -229,6 +239,16 @@ void UContext::residual(bool singles, const double* T1d, const double* T2d, doub
   ops.axpby(n2(), 0.5, T2d, 0.0, g);
   ops.axpby(n2(), 1.0, tt, 1.0, g);
   ops.contract_p(F, -1.0, G, "fbjn", Aoovv, "mnef", 1.0, W2, "mbej");
+  if (dc) {
+    // exchange part of the T2 ring term out: + 1/2 <mn|fe> T2[b,f,j,n],  <mn|fe> = (mf|ne)
+    size_t mk = ops.arena.mark();
+    double* kx = take(n2());
+    Tensor Kx = t4(kx, o, o, v, v);
+    Tensor Bo = t3(Bov.d(), o, v, nq);
+    ops.contract_p(F, 1.0, Bo, "mfQ", Bo, "neQ", 0.0, Kx, "mnef");
+    ops.contract_p(F, 0.5, Kx, "mnef", T2, "bfjn", 1.0, W2, "mbej");
+    ops.arena.release_to(mk);
+  }
   // singles
   if (singles) {
     ops.copy(v * o, fvo.d(), R1d);
@@ -250,6 +270,7 @@ void UContext::residual(bool singles, const double* T1d, const double* T2d, doub
     Tensor H = t2(hb, v, v);
     ops.copy(v * v, fae, hb);
     ops.contract_p(F, -0.5, T1, "bm", Fme, "me", 1.0, H, "be");
+    if (dc) ops.contract_p(F, 0.25, T2, "bfmn", Aoovv, "mnef", 1.0, H, "be");   // x_ab halved
     ops.contract_p(F, 1.0, T2, "aeij", H, "be", 1.0, R2, "abij");
     ops.contract_p(F, -1.0, T2, "beij", H, "ae", 1.0, R2, "abij");
     // - P(ij) T2[a,b,i,m] (F_mj + F_me T1[e,j] / 2)
@@ -257,6 +278,7 @@ void UContext::residual(bool singles, const double* T1d, const double* T2d, doub
     Tensor G2 = t2(g2, o, o);
     ops.copy(o * o, fmi, g2);
     ops.contract_p(F, 0.5, Fme, "me", T1, "ej", 1.0, G2, "mj");
+    if (dc) ops.contract_p(F, -0.25, T2, "efjn", Aoovv, "mnef", 1.0, G2, "mj");  // x_ij halved
     ops.contract_p(F, -1.0, T2, "abim", G2, "mj", 1.0, R2, "abij");
     ops.contract_p(F, 1.0, T2, "abjm", G2, "mi", 1.0, R2, "abij");
   }
@@ -287,6 +309,11 @@ void UContext::residual(bool singles, const double* T1d, const double* T2d, doub
     Tensor X = t4(x, o, o, o, o);
     ops.contract_p(F, 0.5, Aoovv, "mnef", Ttau, "efij", 0.0, X, "mnij");
     ops.contract_p(F, 0.25, Ttau, "abmn", X, "mnij", 1.0, R2, "abij");
+    if (dc) {
+      // hole-hole term out: - 1/4 T2[a,b,m,n] sum_ef <mn||ef> T2[e,f,i,j]
+      ops.contract_p(F, 1.0, Aoovv, "mnef", T2, "efij", 0.0, X, "mnij");
+      ops.contract_p(F, -0.25, T2, "abmn", X, "mnij", 1.0, R2, "abij");
+    }
   }
   {
     // P(ij) P(ab) [T2[a,e,i,m] W_mbej - T1[a,m] T1[e,i] <mb||ej>]
@@ -353,12 +380,13 @@ void UContext::mp2(double* T1d, double* T2d, double* emp2) {
   ops.arena.reset();
 }
 
-// cc_iterations! (cc.jl:2518-2648) for UCCSD / UCCD from the UMP2 guess: residual, Hylleraas
+// cc_iterations! (cc.jl:2518-2648) for UCCSD / UCCD / UDCSD from the UMP2 guess: residual, Hylleraas
 // energy, update with level shifts, DIIS with the reference's unrestricted dots (singles 1,
 // doubles 1/4 per spin-orbital element: calc_u_singles_dot, calc_samespin_doubles_dot,
 // calc_ab_doubles_dot, cctools.jl:453-540), convergence on the squared residual norm and
 // the energy change.  energies = [E_H, E_proj, dE, NormR].
-int UContext::solve(bool singles, const SolverOpts& o, double* energies, int* converged) {
+int UContext::solve(int method, const SolverOpts& o, double* energies, int* converged) {
+  const bool singles = has_singles(method);
   const long m1 = n1(), m2 = n2();
   T1.ensure(sizeof(double) * m1);
   T2.ensure(sizeof(double) * m2);
@@ -386,7 +414,7 @@ int UContext::solve(bool singles, const SolverOpts& o, double* energies, int* co
   int conv = 0, it = 0;
   fp64_iterations = 0;
   for (it = 1; it <= o.maxit; ++it) {
-    residual(singles, T1.d(), T2.d(), R1.d(), R2.d());
+    residual(method, T1.d(), T2.d(), R1.d(), R2.d());
     fp64_iterations++;
     // calc_hylleraas (cc.jl:286-308): projected energy + 1/4 T2.R2 + R1.T1
     Eh = proj_energy(T1.d(), T2.d()) + 0.25 * dot(T2.d(), R2.d(), m2) + dot(R1.d(), T1.d(), m1);

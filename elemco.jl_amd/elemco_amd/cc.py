@@ -196,8 +196,10 @@ def ccdriver(EC: ECInfo, method: str):
 
 
 def ccdriver_unrestricted(fcidump: str, method: str, ms2=None, thr=1e-10, maxit=100):
-    """``ccdriver(EC, method; fcidump)`` for the open-shell methods ``uccsd``, ``uccd``, ``uccsd(t)``
-    (``src/cc/ccdriver.jl:37-76,348-393``; ``@opt wf ms2=...`` is ``ms2``, None keeps the file's MS2).
+    """``ccdriver(EC, method; fcidump)`` for the open-shell methods ``uccsd``, ``uccd``, ``udcsd``
+    (also spelled ``dcsd``, which the reference runs as UDCSD on an open-shell system) and
+    ``uccsd(t)`` (``src/cc/ccdriver.jl:37-76,348-393``; ``@opt wf ms2=...`` is ``ms2``, None keeps
+    the file's MS2).
 
     Returns the reference's energy keys: ``HF``, ``MP2``, ``MP2c``, ``<M>``, ``<M>c`` and for
     ``uccsd(t)`` ``[T]``, ``(T)``, ``UCCSD(T)``, ``UCCSD(T)c``.
@@ -205,8 +207,10 @@ def ccdriver_unrestricted(fcidump: str, method: str, ms2=None, thr=1e-10, maxit=
     m = method.lower()
     pert = m.endswith("(t)")
     base = m[:-3] if pert else m
-    if base not in ("uccsd", "uccd") or (pert and base != "uccsd"):
-        raise lib.ElemcoError(f"method {method} is not on the open-shell path (uccsd, uccd, uccsd(t))")
+    if base == "dcsd":
+        base = "udcsd"   # dcsd on an open-shell system (checkset_unrestricted_closedshell!)
+    if base not in ("uccsd", "uccd", "udcsd") or (pert and base != "uccsd"):
+        raise lib.ElemcoError(f"method {method} is not on the open-shell path (uccsd, uccd, udcsd, uccsd(t))")
     ctx = lib.UContext.from_fcidump(fcidump, -1 if ms2 is None else int(ms2))
     try:
         out = {}
@@ -214,8 +218,8 @@ def ccdriver_unrestricted(fcidump: str, method: str, ms2=None, thr=1e-10, maxit=
         out["HF"] = ehf
         _, emp2 = ctx.mp2()
         out["MP2c"], out["MP2"] = emp2, emp2 + ehf
-        e, _, it, conv = ctx.solve(lib.ECG_UCCSD if base == "uccsd" else lib.ECG_UCCD,
-                                   lib.default_opts(thr=thr, maxit=maxit))
+        code = {"uccsd": lib.ECG_UCCSD, "uccd": lib.ECG_UCCD, "udcsd": lib.ECG_UDCSD}[base]
+        e, _, it, conv = ctx.solve(code, lib.default_opts(thr=thr, maxit=maxit))
         if not conv:
             print("WARNING: CC iterations did not converge!")
         name = base.upper()

@@ -95,7 +95,7 @@ _SIGS.update({
     "ecg_ucc_solve": [_vp, ctypes.c_int, ctypes.POINTER(SolverOpts)] + [_dp] * 6 + [_ip, _ip],
     "ecg_upert_triples": [_vp] + [_dp] * 7,
 })
-ECG_UCCSD, ECG_UCCD = 0, 1
+ECG_UCCSD, ECG_UCCD, ECG_UDCSD = 0, 1, 2
 _RESTYPES = {"ecg_default_opts": None, "ecg_last_error": ctypes.c_char_p, "ecg_ulast_error": ctypes.c_char_p}
 EXPORTED = tuple(_SIGS)
 

@@ -265,9 +265,10 @@ int ecg_one_rdm(ecg_ctx* ctx, const double* T1, const double* T2, const double* 
 int ecg_profile(ecg_ctx* ctx, int enable);
 int ecg_stats(ecg_ctx* ctx, double* stats);
 
-/* ---- Open-shell (unrestricted) coupled cluster: UCCSD (ECG_UCCSD) and UCCD (ECG_UCCD, no
- * singles).  Replaces the unrestricted branch of calc_cc (src/cc/cc.jl:2451-2516) with its
- * residual calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab; dc=false) (src/cc/cc.jl:1496-1934),
+/* ---- Open-shell (unrestricted) coupled cluster: UCCSD (ECG_UCCSD), UCCD (ECG_UCCD, no
+ * singles) and UDCSD (ECG_UDCSD).  Replaces the unrestricted branch of calc_cc
+ * (src/cc/cc.jl:2451-2516) with its residual calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab; dc)
+ * (src/cc/cc.jl:1496-1934),
  * calc_UMP2 (cc.jl:1034-1060) and the open-shell Fock / HF energy (calc_fock_matrix,
  * calc_HF_energy, src/cc/cctools.jl:35-76).  The alpha spin orbitals are the lowest nocca
  * orbitals, the beta ones the lowest noccb (the default spaces o / O of setup_space_fd!,
@@ -276,7 +277,7 @@ int ecg_stats(ecg_ctx* ctx, double* stats);
  * antisymmetric), T2ab [va, vb, na, nb], with na = nocca, nb = noccb, va = norb - na,
  * vb = norb - nb.  Hermitian integrals, (pq|rs) = sum_Q B[p,q,Q] B[r,s,Q].  Status codes and
  * errors as for the closed-shell context (message: ecg_ulast_error). */
-enum { ECG_UCCSD = 0, ECG_UCCD = 1 };
+enum { ECG_UCCSD = 0, ECG_UCCD = 1, ECG_UDCSD = 2 };
 typedef struct ecg_uctx ecg_uctx;
 int ecg_ucreate(ecg_uctx** ctx, int nocca, int noccb, int norb, int naux, const double* h, const double* B,
                 double e0, int device, size_t workspace_bytes);

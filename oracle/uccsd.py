@@ -26,8 +26,9 @@ Around it, restated from the reference:
     DIIS (``oracle.ccsd.Diis``) on plain dots.
 
 Pins (tests/test_oracle.py): the closed-shell limit (nocca = noccb) equals the
-closed-shell oracle's residual block by block for random amplitudes; the
-reference's open-shell energies of ``test/h2o_cation.jl`` (HF, MP2, UCCSD) and
+closed-shell oracle's CCSD and DCSD residuals block by block for random amplitudes;
+the reference's open-shell energies of ``test/h2o_cation.jl`` (HF, MP2, UCCSD, UCCD,
+UDCSD) and
 ``test/h2o_triplet.jl`` (HF, MP2, UCCSD); two same-spin electrons, where UCCSD
 is full CI of the Sz = 1 space (closed form: a dense eigenproblem).
 """
@@ -65,6 +66,7 @@ class USystem:
         g = eri[np.ix_(sp, sp, sp, sp)].transpose(0, 2, 1, 3)
         g = g * same[:, None, :, None] * same[None, :, None, :]
         self.A = g - g.transpose(0, 1, 3, 2)
+        self.g = g
         self.o = slice(0, self.O)
         self.v = slice(self.O, self.O + self.V)
         # gen_fock(EC, :alpha / :beta): f = h + sum_m <pm||qm> over occupied spin orbitals
@@ -144,9 +146,20 @@ def _p(X, axes):
     return X - X.transpose(perm)
 
 
-def calc_ucc_resid(S: USystem, T1, T2):
-    """UCCSD residual (R1[a,i], R2[a,b,i,j]) on spin orbitals (the equations of the reference's
-    ``calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab)``, ``cc.jl:1496-1934``, dc = false)."""
+def calc_ucc_resid(S: USystem, T1, T2, dc=False):
+    """UCCSD (dc = False) or UDCSD (dc = True) residual (R1[a,i], R2[a,b,i,j]) on spin orbitals
+    (the equations of the reference's ``calc_cc_resid(EC, T1a, T1b, T2a, T2b, T2ab; dc)``,
+    ``cc.jl:1496-1934``).
+
+    The reference writes both in T1-dressed form: the doubles residual is the dressed linear part
+    plus terms quadratic in T2 with the undressed <kl|cd>, and the distinguishable-cluster
+    variant changes only those quadratic terms (``cc.jl:1724-1808``): the hole-hole term
+    (``x_klij``, ``x_KLIJ``, ``x_kLiJ`` += oovv T2) is dropped, the quadratic parts of
+    ``x_ij`` / ``x_ab`` are halved (``dcfac``), and the ring terms keep only the integral
+    <kl|cd> whose pairs (k,c), (l,d) each sit in one amplitude (the exchange parts
+    ``oovv[k,l,d,c]`` and the crossed ``x_DBik`` term are dropped).  Since the singles and the
+    linear part are the CCSD ones, R_DC = R_CCSD - dQ with dQ the difference of those
+    quadratic terms, written here in spin orbitals (``_dc_correction``)."""
     o, v = S.o, S.v
     f = S.f
     foo, fvv, fov, fvo = f[o, o], f[v, v], f[o, v], f[v, o]
@@ -186,7 +199,24 @@ def calc_ucc_resid(S: USystem, T1, T2):
     R2 += _p(_p(X, (0, 1)), (2, 3))
     R2 += _p(_es('ei,abej->abij', T1, vvvo), (2, 3))
     R2 -= _p(_es('am,mbij->abij', T1, ovoo), (0, 1))
+    if dc:
+        R2 -= _dc_correction(S, T2)
     return R1, R2
+
+
+def _dc_correction(S: USystem, T2):
+    """CCSD minus DCSD quadratic doubles terms (T2 only, undressed <mn||ef> = <mn|ef> - <mn|fe>):
+    1/4 t_mn^ab t_ij^ef <mn||ef>  (hole-hole, dropped)
+    - 1/4 P(ab) t_ij^ae t_mn^bf <mn||ef> - 1/4 P(ij) t_im^ab t_jn^ef <mn||ef>  (x_ab, x_ij halved)
+    - 1/2 P(ij) P(ab) t_im^ae t_jn^bf <mn|fe>  (exchange ring, dropped)."""
+    o, v = S.o, S.v
+    oovv = S.ints('oovv')
+    ex = S.g[o, o, v, v].transpose(0, 1, 3, 2)      # <mn|fe> as [m,n,e,f]
+    dQ = 0.25 * _es('abmn,efij,mnef->abij', T2, T2, oovv)
+    dQ -= 0.25 * _p(_es('aeij,bfmn,mnef->abij', T2, T2, oovv), (0, 1))
+    dQ -= 0.25 * _p(_es('abim,efjn,mnef->abij', T2, T2, oovv), (2, 3))
+    dQ -= 0.5 * _p(_p(_es('aeim,bfjn,mnef->abij', T2, T2, ex), (0, 1)), (2, 3))
+    return dQ
 
 
 def update(S: USystem, R1, R2, shifts, shiftp):
@@ -202,9 +232,9 @@ class _Opts:
         self.options = dict(thr=thr, conven=0.1, maxit=maxit, shifts=0.15, shiftp=0.2, maxdiis=6, resthr=10.0)
 
 
-def calc_ucc(S: USystem, thr=1e-10, maxit=100, singles=True):
+def calc_ucc(S: USystem, thr=1e-10, maxit=100, singles=True, dc=False):
     """``cc_iterations!`` (``cc.jl:2518-2648``) for UCCSD (UCCD with ``singles=False``: T1 = 0,
-    ``exclevel[1] = :none``) from the UMP2 guess: residual, Hylleraas energy, update with
+    ``exclevel[1] = :none``; UDCSD with ``dc=True``) from the UMP2 guess: residual, Hylleraas energy, update with
     shifts, DIIS, convergence on the squared residual norm (``calc_singles_norm`` /
     ``calc_doubles_norm`` of the spin blocks) and the energy change.  Returns
     (E_H, T1, T2, converged, iterations)."""
@@ -219,7 +249,7 @@ def calc_ucc(S: USystem, thr=1e-10, maxit=100, singles=True):
     thren = np.sqrt(thr) * 0.1
     Eh, conv, it = 0.0, False, 0
     for it in range(1, maxit + 1):
-        R1, R2 = calc_ucc_resid(S, T1, T2)
+        R1, R2 = calc_ucc_resid(S, T1, T2, dc=dc)
         if not singles:
             R1[...] = 0.0
         Eh = calc_hylleraas(S, T1, T2, R1, R2)
@@ -236,13 +266,15 @@ def calc_ucc(S: USystem, thr=1e-10, maxit=100, singles=True):
 
 
 def ucc_driver(h, eri, e0, nocca, noccb, thr=1e-10, method="uccsd"):
-    """Energies with the reference's keys for ``@cc uccsd`` / ``@cc uccd`` (``ccdriver``,
-    ``src/cc/ccdriver.jl:37-76``): HF, MP2 (total), UCCSDc (correlation) and UCCSD (total)."""
+    """Energies with the reference's keys for ``@cc uccsd`` / ``@cc uccd`` / ``@cc udcsd``
+    (``ccdriver``, ``src/cc/ccdriver.jl:37-76``; ``dcsd`` on an open-shell system is UDCSD,
+    ``checkset_unrestricted_closedshell!``, ``ccdriver.jl:202-214``): HF, MP2 (total), UCCSDc
+    (correlation) and UCCSD (total)."""
     S = USystem(h, eri, e0, nocca, noccb)
     EHF = hf_energy(S)
     _, _, Emp2 = calc_UMP2(S)
     name = method.upper()
-    E, T1, T2, conv, it = calc_ucc(S, thr=thr, singles=(method == "uccsd"))
+    E, T1, T2, conv, it = calc_ucc(S, thr=thr, singles=(method != "uccd"), dc=(method == "udcsd"))
     return {'HF': EHF, 'MP2c': Emp2, 'MP2': EHF + Emp2, name + 'c': E, name: EHF + E, 'converged': conv,
             'iterations': it}
 

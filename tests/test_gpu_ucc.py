@@ -1,11 +1,11 @@
-"""Open-shell UCCSD / UCCD on the GPU (ecg_ucreate*, ecg_ucc_residual, ecg_ucc_solve) against
+"""Open-shell UCCSD / UCCD / UDCSD on the GPU (ecg_ucreate*, ecg_ucc_residual, ecg_ucc_solve) against
 the CPU oracle (oracle/uccsd.py) and the reference's own open-shell test energies.
 
 Bars: residual spin blocks within 1e-10 (max-abs) of the oracle for random amplitudes;
 UMP2 amplitudes 1e-12; HF / MP2 energies 1e-10 of the oracle; converged UCCSD / UCCD
 energies within 1e-9 Eh of the oracle solved to the same threshold and 1e-6 Eh of
 test/h2o_cation.jl and test/h2o_triplet.jl; the closed-shell limit equals the closed-shell
-GPU path's CCSD energy (1e-9 Eh).
+GPU path's CCSD (DCSD) energy (1e-9 Eh).
 """
 import os
 
@@ -61,6 +61,12 @@ def test_ucc_residual_fock_mp2_match_oracle(name, ms2):
         assert np.max(np.abs(Rgd[0])) == 0.0 and np.max(np.abs(Rgd[1])) == 0.0
         for g, ref in zip(Rgd[2:], Rod[2:]):
             assert np.max(np.abs(g - ref)) < 1e-10
+        # UDCSD: every distinguishable-cluster change contributes for random amplitudes
+        Rodc = S.to_blocks(*U.calc_ucc_resid(S, T1s, T2s, dc=True))
+        Rgdc = ctx.residual(blocks, E.ECG_UDCSD)
+        for k, (g, ref) in enumerate(zip(Rgdc, Rodc)):
+            assert np.max(np.abs(g - ref)) < 1e-10, (k, np.max(np.abs(g - ref)))
+        assert np.max(np.abs(Rgdc[4] - Rg[4])) > 1e-4
     finally:
         ctx.close()
 
@@ -68,15 +74,16 @@ def test_ucc_residual_fock_mp2_match_oracle(name, ms2):
 @pytest.mark.parametrize("name,ms2,method,eref", [
     ("H2O_CATION.FCIDUMP.gz", 1, "uccsd", -0.232320803220),
     ("H2O_CATION.FCIDUMP.gz", 1, "uccd", -0.213495204514),
+    ("H2O_CATION.FCIDUMP.gz", 1, "udcsd", -0.243223819179),
     ("H2O.FCIDUMP.gz", 2, "uccsd", -0.276560985613),
 ])
 def test_ucc_solve_matches_oracle_and_reference(name, ms2, method, eref):
     S, fd = _oracle(name, ms2)
-    eo, _, _, conv_o, _ = U.calc_ucc(S, thr=1e-14, singles=(method == "uccsd"))
+    eo, _, _, conv_o, _ = U.calc_ucc(S, thr=1e-14, singles=(method != "uccd"), dc=(method == "udcsd"))
     assert conv_o
     ctx = E.UContext.from_fcidump(os.path.join(GOLDEN, name), ms2=ms2)
     try:
-        m = E.ECG_UCCSD if method == "uccsd" else E.ECG_UCCD
+        m = {"uccsd": E.ECG_UCCSD, "uccd": E.ECG_UCCD, "udcsd": E.ECG_UDCSD}[method]
         e, T, it, conv = ctx.solve(m, E.default_opts(thr=1e-14, maxit=100))
         assert conv, it
         print(f"\n{name} {method}: E = {e[0]:.12f} (oracle {eo:.12f}, reference {eref:.12f}) it = {it}", flush=True)
@@ -86,15 +93,17 @@ def test_ucc_solve_matches_oracle_and_reference(name, ms2, method, eref):
         ctx.close()
 
 
-def test_ucc_closed_shell_limit_equals_ccsd():
-    """nocca = noccb on the closed-shell H2O fixture: the UCCSD energy is the closed-shell CCSD one."""
+@pytest.mark.parametrize("um,cm", [("ECG_UCCSD", "ECG_CCSD"), ("ECG_UDCSD", "ECG_DCSD")])
+def test_ucc_closed_shell_limit_equals_ccsd(um, cm):
+    """nocca = noccb on the closed-shell H2O fixture: the UCCSD (UDCSD) energy is the closed-shell
+    CCSD (DCSD) one."""
     path = os.path.join(GOLDEN, "H2O.FCIDUMP.gz")
     ctx = E.UContext.from_fcidump(path, ms2=0)
     cs = E.Context.from_fcidump(path)
     try:
         opts = E.default_opts(thr=1e-14, maxit=100)
-        eu = ctx.solve(E.ECG_UCCSD, opts)[0][0]
-        ec = cs.cc_solve(E.ECG_CCSD, E.ECG_FP64, opts)[0][0]
+        eu = ctx.solve(getattr(E, um), opts)[0][0]
+        ec = cs.cc_solve(getattr(E, cm), E.ECG_FP64, opts)[0][0]
         assert abs(eu - ec) < 1e-9, (eu, ec)
     finally:
         ctx.close()
@@ -130,3 +139,11 @@ def test_ccdriver_unrestricted_reference_keys():
            "UCCSD": -0.276560985613 - 75.62407982361415, "UCCSD(T)": -0.295388574918 - 75.62407982361415}
     for k, v in ref.items():
         assert abs(e[k] - v) < 1e-6, (k, e[k], v)
+
+
+def test_ccdriver_dcsd_on_open_shell_is_udcsd():
+    """``@cc dcsd`` on the open-shell H2O+ dump (test/h2o_cation.jl:27-28) runs UDCSD: 1e-6 of the
+    reference's EDCSD."""
+    e = E.ccdriver_unrestricted(os.path.join(GOLDEN, "H2O_CATION.FCIDUMP.gz"), "dcsd")
+    assert e["converged"]
+    assert abs(e["UDCSDc"] - (-0.243223819179)) < 1e-6

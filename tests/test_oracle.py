@@ -305,10 +305,12 @@ def _uccsd_system(name, ms2):
     return U, U.USystem(fd.int1, fd.int2.transpose(0, 2, 1, 3), fd.int0, na, nb), fd
 
 
-def test_uccsd_closed_shell_limit_equals_ccsd_residual():
-    """With nocca = noccb the spin-orbital UCCSD residual's blocks are the closed-shell residual
-    (calc_cc_resid, cc.jl:1343-1489): R1a = R1b = R1, R2ab = R2, R2a = R2b = R2 - R2[b,a,i,j], for
-    random amplitudes (every term of both residuals contributes)."""
+@pytest.mark.parametrize("dc", [False, True])
+def test_uccsd_closed_shell_limit_equals_ccsd_residual(dc):
+    """With nocca = noccb the spin-orbital UCCSD (UDCSD) residual's blocks are the closed-shell
+    CCSD (DCSD) residual (calc_cc_resid, cc.jl:1343-1489): R1a = R1b = R1, R2ab = R2,
+    R2a = R2b = R2 - R2[b,a,i,j], for random amplitudes (every term of both residuals
+    contributes; a dropped or mis-signed term of the distinguishable-cluster correction fails)."""
     U, S, fd = _uccsd_system("H2O.FCIDUMP.gz", 0)
     EC = ECInfo.from_fcidump(fd)
     calc_fock_matrix(EC)
@@ -318,9 +320,9 @@ def test_uccsd_closed_shell_limit_equals_ccsd_residual():
     T1 = 0.05 * rng.standard_normal((nv, no))
     T2 = 0.05 * rng.standard_normal((nv, nv, no, no))
     T2 = T2 + T2.transpose(1, 0, 3, 2)
-    R1c, R2c = calc_cc_resid(EC, T1, T2)
+    R1c, R2c = calc_cc_resid(EC, T1, T2, dc=dc)
     Ts = S.from_blocks(T1, T1, T2 - T2.transpose(1, 0, 2, 3), T2 - T2.transpose(1, 0, 2, 3), T2)
-    R1a, R1b, R2a, R2b, R2ab = S.to_blocks(*U.calc_ucc_resid(S, *Ts))
+    R1a, R1b, R2a, R2b, R2ab = S.to_blocks(*U.calc_ucc_resid(S, *Ts, dc=dc))
     R2s = R2c - R2c.transpose(1, 0, 2, 3)
     for got, ref in ((R1a, R1c), (R1b, R1c), (R2ab, R2c), (R2a, R2s), (R2b, R2s)):
         assert np.max(np.abs(got - ref)) < 1e-12
@@ -329,10 +331,12 @@ def test_uccsd_closed_shell_limit_equals_ccsd_residual():
 @pytest.mark.parametrize("name,ms2,method,ref", [
     ("H2O_CATION.FCIDUMP.gz", 1, "uccsd", {"HF": -75.337282954481, "MP2c": -0.207727619864, "UCCSDc": -0.232320803220}),
     ("H2O_CATION.FCIDUMP.gz", 1, "uccd", {"UCCDc": -0.213495204514}),
+    ("H2O_CATION.FCIDUMP.gz", 1, "udcsd", {"UDCSDc": -0.243223819179}),
     ("H2O.FCIDUMP.gz", 2, "uccsd", {"HF": -75.62407982361415, "MP2c": -0.22401008330020, "UCCSDc": -0.276560985613}),
 ])
 def test_uccsd_reference_energies(name, ms2, method, ref):
-    """test/h2o_cation.jl:4-26 (H2O+ doublet on ROHF orbitals: HF, MP2, UCCSD, UCCD) and
+    """test/h2o_cation.jl:4-29 (H2O+ doublet on ROHF orbitals: HF, MP2, UCCSD, UCCD, and DCSD,
+    which the driver runs as UDCSD on an open-shell system) and
     test/h2o_triplet.jl:4-21 (H2O triplet, ms2 = 2: HF, MP2, UCCSD), at the reference's 1e-6."""
     from oracle import uccsd as U
     fd = read_fcidump(os.path.join(GOLDEN, name))
