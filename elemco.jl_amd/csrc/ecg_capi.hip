@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -87,6 +88,30 @@ int uguarded(ecg_uctx* ctx, F&& f) {
 
 void check_umethod(int m) {
   ECG_CHECK(m == ECG_UCCSD || m == ECG_UCCD || m == ECG_UDCSD, "method must be ECG_UCCSD, ECG_UCCD or ECG_UDCSD");
+}
+
+// Region of the allocation-free entry point: one copy per GPU, held from the
+// library's first use of that GPU until the process ends.  Host code never passes
+// `scratch` itself to a kernel or a copy (on the host the name only stands for the
+// symbol): it uses the device address looked up for the current device.
+static __device__ __align__(256) unsigned char scratch[ECG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device; looked
+// up on the first call for that device and kept.
+void* scratch_on(int device) {
+  ECG_CHECK(device >= 0 && device < kMaxDevices, "device index out of range for the allocation-free scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    ECG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ECG_CHECK(p != nullptr, "hipGetSymbolAddress returned no address for the allocation-free scratch");
+    g_scratch_dev[device] = p;
+  }
+  return g_scratch_dev[device];
 }
 
 void h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
@@ -450,6 +475,18 @@ int ecg_cc_residual_dev(ecg_ctx* ctx, int method, int precision, const double* d
     check_method(method);
     ECG_CHECK(dT1 && dT2 && dR1 && dR2, "NULL device pointer");
     c.residual(method, to_prec(precision), dT1, dT2, dR1, dR2);
+    ECG_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int ecg_cc_residual_dev_noalloc(ecg_ctx* ctx, int method, int precision, const double* dT1, const double* dT2,
+                                double* dR1, double* dR2) {
+  return guarded(ctx, [&](ecg::Context& c) {
+    check_method(method);
+    ECG_CHECK(precision == ECG_FP64, "the allocation-free residual computes in FP64 only (precision ECG_FP64)");
+    ECG_CHECK(dT1 && dT2 && dR1 && dR2, "NULL device pointer");
+    void* region = scratch_on(c.device);   // guarded() made c.device current
+    c.residual_noalloc(method, dT1, dT2, dR1, dR2, region, ECG_NOALLOC_SCRATCH_BYTES);
     ECG_HIP(hipStreamSynchronize(c.stream));
   });
 }

@@ -56,6 +56,13 @@ struct Context {
   void build_fock();
   void dress(const double* T1d);                       // T1 on device
   void residual(int method, Prec pr, const double* T1d, const double* T2d, double* R1d, double* R2d);
+  // The FP64 residual with every contraction temporary and split-K partial sum taken
+  // from the caller's device region [scratch, scratch + bytes): nothing is allocated
+  // (the dressed factors are allocated with the context).  Throws if the system needs
+  // more than the region holds.
+  void residual_noalloc(int method, const double* T1d, const double* T2d, double* R1d, double* R2d, void* scratch,
+                        size_t bytes);
+  size_t residual_noalloc_min_bytes() const;   // lower bound of the region that call needs
   // adds the (a <= b half of the) DF particle-particle ladder to R2r; T2x: amplitudes as [i,j,c,d]
   void ladder(Prec pr, const double* T2x, double* R2r);
   void ladder_df(Prec pr, const double* Lf, const double* Rf, const double* T2x, double* R2r);

@@ -158,6 +158,18 @@ void Context::init(int nocc, int nvirt, int naux, const double* h_host, const do
   };
   load_factor(BL_host, L);
   if (!hermitian) load_factor(BR_host, R);
+  // dressed factors and integrals: allocated with the context, rewritten by every residual
+  auto alloc_dressed = [&](Blocks& D) {
+    D.oo.alloc(sizeof(double) * (size_t)no * no * nq);
+    D.ov.alloc(sizeof(double) * (size_t)no * nv * nq);
+    D.vo.alloc(sizeof(double) * (size_t)nv * no * nq);
+    D.vv.alloc(sizeof(double) * (size_t)nv * nv * nq);
+  };
+  alloc_dressed(Ld);
+  if (!hermitian) alloc_dressed(Rd);
+  dh.alloc(sizeof(double) * (size_t)n * n);
+  df.alloc(sizeof(double) * (size_t)n * n);
+  trace.alloc(sizeof(double) * (size_t)nq);
   // workspace: contraction temporaries and ladder blocks
   long ovov = nvvoo();
   size_t want = sizeof(double) * (size_t)(18 * ovov + 4L * nv * nv * nq + 64L * 1024 * 1024);
@@ -317,7 +329,7 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
 void df_ladder_half(TensorOps& ops, long nv, long no, int nq, Prec pr, const double* Lf, const double* Rf,
                     const double* T2x, double* R2r) {
   size_t mark = ops.arena.mark();
-  size_t free_b = ops.arena.buf.bytes - ops.arena.mark();
+  size_t free_b = ops.arena.free_bytes();
   const long v = nv;
   const long per_a = v * v * v;
   long ab = std::max<long>(1, std::min<long>(v, (long)(free_b * 0.8 / sizeof(double)) / per_a));
@@ -490,6 +502,41 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   ops.arena.reset();
 }
 
+// Lower bound of the caller region residual_noalloc needs: the residual's own
+// temporaries at their peak (five o^2 v^2 tensors, d_oovo, Z[c,i,Q], W[k,l,i,j],
+// x_ad, x_ki, f_ov) and one a-row (v^3) of the ladder's integral block.  The
+// contraction planner's transposed operands come on top; if they do not fit, the
+// arena stops the call with an error before any launch that would overrun it.
+size_t Context::residual_noalloc_min_bytes() const {
+  const long v = nv, o = no, Q = nq;
+  const long d = 5 * nvvoo() + o * o * o * v + v * o * Q + o * o * o * o + v * v + o * o + o * v + v * v * v;
+  return sizeof(double) * (size_t)d + 16 * 256;
+}
+
+void Context::residual_noalloc(int method, const double* T1d, const double* T2d, double* R1d, double* R2d,
+                               void* scratch, size_t bytes) {
+  const size_t need = residual_noalloc_min_bytes();
+  if (need > bytes)
+    throw Error(-2, "system too large for the allocation-free residual: needs at least " +
+                        std::to_string(need >> 20) + " MiB of scratch, the region holds " +
+                        std::to_string(bytes >> 20) + " MiB");
+  // the arena and the engine's split-K partials run from the caller's region for this call
+  struct Restore {
+    TensorOps& ops;
+    ~Restore() {
+      ops.arena.ext = nullptr;
+      ops.arena.ext_bytes = 0;
+      ops.arena.reset();
+      ops.gemm.transient = nullptr;
+    }
+  } restore{ops};
+  ops.arena.ext = static_cast<char*>(scratch);
+  ops.arena.ext_bytes = bytes;
+  ops.arena.reset();
+  ops.gemm.transient = &ops.arena;
+  residual(method, Prec::F64, T1d, T2d, R1d, R2d);
+}
+
 // The residual from a captured HIP graph: the first call for a (method, precision,
 // buffers) key runs eagerly (sizing every scratch buffer and producing this call's
 // result) and then records the same launch sequence; later calls replay it.  The
@@ -500,7 +547,7 @@ void Context::residual_graph(int method, Prec pr, const double* T1d, const doubl
     g.ws = ops.gemm.ws.p; g.wsb = ops.gemm.ws.bytes;
     g.pk = ops.gemm.pk.p; g.pkb = ops.gemm.pk.bytes;
     g.keep = ops.gemm.pk_keep.p; g.keepb = ops.gemm.pk_keep.bytes;
-    g.arena = ops.arena.buf.p;
+    g.arena = ops.arena.base();
   };
   for (size_t k = 0; k < rgraphs.size(); ++k) {
     ResidualGraph& g = rgraphs[k];

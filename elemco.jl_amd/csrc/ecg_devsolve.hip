@@ -328,7 +328,7 @@ void Context::dev_launch(int method, const SolverOpts& o) {
   DevIter& d = dv;
   auto key_now = [&]() {
     std::vector<const void*> k{T1.p, T2.p, R1.p, R2.p, d.scal.p, d.part.p, d.tt.p, d.fvo.p, ops.gemm.ws.p,
-                               ops.gemm.pk.p, ops.gemm.pk_keep.p, ops.arena.buf.p, ops.red.p};
+                               ops.gemm.pk.p, ops.gemm.pk_keep.p, ops.arena.base(), ops.red.p};
     for (int i = 0; i < d.maxdiis; ++i) {
       k.push_back(diis.amps[i].p);
       k.push_back(diis.res[i].p);

@@ -22,6 +22,7 @@
 namespace ecg {
 
 constexpr int MAXSEG = 4;
+struct Arena;
 
 struct GemmSeg {
   const double* A = nullptr; long a_k = 0;
@@ -144,7 +145,11 @@ struct GemmEngine {
   // position 8*(k%4) + k/4 holds k (the order the FP32 packed product reads).
   void pack_rows_f32(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad, float* out);
   void pack_rows_f64(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad, double* out);
-  void reduce_slabs(const GemmOp& op, int ksplit);   // split-K partial slabs in ws -> C
+  void reduce_slabs(const GemmOp& op, int ksplit, const double* wsp);   // split-K partial slabs -> C
+  // While set, split-K partial sums are taken from this region (and returned after each
+  // product) instead of `ws`, no operand is packed into planes, and only FP64 products
+  // run: the engine then allocates nothing (the allocation-free entry points).
+  Arena* transient = nullptr;
   hipEvent_t timing_begin();
   void timing_end(hipEvent_t e0);
 };

@@ -8,6 +8,7 @@
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16*(i%4), slot i/4.
 #include "ecg_gemm.h"
 #include "ecg_gemm_dev.h"
+#include "ecg_tensor.h"
 
 #include <algorithm>
 
@@ -298,7 +299,8 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     }
     pr = Prec::F32;
   }
-  if (pr == Prec::F64 && f64_kernel == 1 && op.batch == 1 &&
+  ECG_CHECK(!transient || pr == Prec::F64, "gemm: products into a caller's scratch region are FP64 only");
+  if (pr == Prec::F64 && f64_kernel == 1 && op.batch == 1 && !transient &&
       (double)op.M * op.N / ((double)op.M + op.N) >= 1024.0) {
     // large FP64 products: operands packed into FP64 planes (16 bytes moved per element)
     // for the direct-to-LDS FP64 kernel.  The packing costs about 100 / (M N / (M + N)) of
@@ -344,14 +346,27 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
     ksplit = (int)std::min<long>((1024 + tiles - 1) / tiles, p.K / (BKS * 8));
     ksplit = std::max(1, std::min(ksplit, 64));
   }
+  long slab = (long)op.M * op.N;
+  size_t tmark = 0;
+  if (transient) {
+    // partial slabs come from the caller's region: split only as far as they fit there
+    tmark = transient->mark();
+    const size_t per = sizeof(double) * (size_t)slab * op.batch + 256;
+    ksplit = (int)std::max<long>(1, std::min<long>(ksplit, (long)(transient->free_bytes() / per)));
+  }
   int kchunk = (p.K + ksplit - 1) / ksplit;
   kchunk = ((kchunk + BKS - 1) / BKS) * BKS;
   ksplit = (p.K + kchunk - 1) / kchunk;
   p.kchunk = kchunk; p.ksplit = ksplit;
-  long slab = (long)op.M * op.N;
+  double* wsp = nullptr;
   if (ksplit > 1) {
-    ws.ensure(sizeof(double) * slab * ksplit * op.batch);
-    p.C = ws.d(); p.cplain = 1; p.ldc_ws = op.M; p.c_z = slab;
+    if (transient) {
+      wsp = transient->take(slab * ksplit * op.batch);
+    } else {
+      ws.ensure(sizeof(double) * slab * ksplit * op.batch);
+      wsp = ws.d();
+    }
+    p.C = wsp; p.cplain = 1; p.ldc_ws = op.M; p.c_z = slab;
   } else {
     p.C = op.C; p.cplain = 0;
     p.c_ms0 = op.c_ms0; p.c_ms1 = op.c_ms1; p.c_md0 = op.c_md0;
@@ -371,7 +386,8 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
   else launch_gemm<float>(ak, bk, p.nseg > 1, grid, stream, p);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
-  if (ksplit > 1) reduce_slabs(op, ksplit);
+  if (ksplit > 1) reduce_slabs(op, ksplit, wsp);
+  if (transient) transient->release_to(tmark);
 }
 
 // Event pair around one GEMM kernel launch when timing is on (pairs are read
@@ -393,10 +409,10 @@ void GemmEngine::timing_end(hipEvent_t e0) {
   if (e0) ECG_HIP(hipEventRecord(evpool[evused - 1], stream));
 }
 
-void GemmEngine::reduce_slabs(const GemmOp& op, int ksplit) {
+void GemmEngine::reduce_slabs(const GemmOp& op, int ksplit, const double* wsp) {
   const long slab = (long)op.M * op.N;
   dim3 g2((unsigned)((slab + 255) / 256), op.batch);
-  hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, ws.d(), ksplit, slab, op.M, op.N, op.C,
+  hipLaunchKernelGGL(splitk_reduce_kernel, g2, dim3(256), 0, stream, wsp, ksplit, slab, op.M, op.N, op.C,
                      op.c_ms0, op.c_ms1, op.c_md0, op.c_ns0, op.c_ns1, op.c_nd0, op.c_z, op.alpha, op.beta);
   ECG_HIP(hipGetLastError());
 }

@@ -866,6 +866,7 @@ void GemmEngine::run_f64p(const GemmOp& op) { run_planes(op, 2); }
 // plane), then the packed-plane kernel; split K when the tiles alone cannot fill the GPU.
 void GemmEngine::run_planes(const GemmOp& op, int kind) {
   ECG_CHECK(op.batch == 1, "gemm: batched packed products are not supported");
+  ECG_CHECK(!transient, "gemm: no packed products into a caller's scratch region");
   GemmSeg one;
   one.A = op.A; one.a_k = op.a_k; one.B = op.B; one.b_k = op.b_k; one.K = op.K;
   const GemmSeg* sg = op.nseg > 0 ? op.seg : &one;
@@ -984,7 +985,7 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
     hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), grid, dim3(512), 0, stream, x);
   ECG_HIP(hipGetLastError());
   timing_end(e0);
-  if (ksplit > 1) reduce_slabs(op, ksplit);
+  if (ksplit > 1) reduce_slabs(op, ksplit, ws.d());
 }
 
 void GemmEngine::pack_rows_bf16x3(const double* src, long rows, int d0, long s0, long s1, int K, long sk, int kpad,

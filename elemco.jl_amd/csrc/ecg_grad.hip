@@ -169,7 +169,7 @@ void energy_grad(Context& c, int method, const double* T1h, const double* T2h, c
     const int p_q[3] = {2, 0, 1};
     ops.permute(lvv, p_q, tLT, 1.0, 0.0);
     if (!herm) ops.permute(rvv, p_q, tRT, 1.0, 0.0);
-    size_t free_b = ops.arena.buf.bytes - ops.arena.mark();
+    size_t free_b = ops.arena.free_bytes();
     const long per_a = v * v * v;
     long ab = std::max<long>(1, std::min<long>(v, (long)(free_b * 0.8 / sizeof(double)) / per_a));
     size_t mark = ops.arena.mark();

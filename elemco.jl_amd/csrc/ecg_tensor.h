@@ -38,15 +38,22 @@ struct Tensor {
   }
 };
 
-// Scratch arena: bump allocation from one device block, reset per phase.
+// Scratch arena: bump allocation from one device block, reset per phase.  The
+// block is the library's own buffer, or, while `ext` is set, a caller-supplied
+// region (the allocation-free entry points), which is never grown or freed.
 struct Arena {
   DevBuf buf;
+  char* ext = nullptr;
+  size_t ext_bytes = 0;
   size_t top = 0;
   size_t high = 0;
   double* take(long n);
   void reset() { top = 0; }
   size_t mark() const { return top; }
   void release_to(size_t m) { top = m; }
+  char* base() const { return ext ? ext : static_cast<char*>(buf.p); }
+  size_t capacity() const { return ext ? ext_bytes : buf.bytes; }
+  size_t free_bytes() const { return capacity() > top ? capacity() - top : 0; }
 };
 
 struct TensorOps {

@@ -10,8 +10,9 @@ namespace ecg {
 // ------------------------------------------------------------------ arena
 double* Arena::take(long n) {
   size_t bytes = ((size_t)std::max<long>(n, 1) * sizeof(double) + 255) & ~(size_t)255;
-  ECG_CHECK(top + bytes <= buf.bytes, "scratch arena exhausted (raise the workspace size)");
-  double* p = reinterpret_cast<double*>(static_cast<char*>(buf.p) + top);
+  ECG_CHECK(top + bytes <= capacity(), ext ? "no-alloc scratch exhausted (the system is too large for it)"
+                                            : "scratch arena exhausted (raise the workspace size)");
+  double* p = reinterpret_cast<double*>(base() + top);
   top += bytes;
   high = std::max(high, top);
   return p;

@@ -43,6 +43,7 @@ _SIGS = {
     "ecg_mp2": [_vp, _dp, _dp, _dp],
     "ecg_cc_residual": [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp],
     "ecg_cc_residual_dev": [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp],
+    "ecg_cc_residual_dev_noalloc": [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp],
     "ecg_cc_state_init": [_vp, _dp, _dp, ctypes.POINTER(SolverOpts)],
     "ecg_cc_state_step": [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SolverOpts), _dp],
     "ecg_cc_state_get": [_vp, _dp, _dp],
@@ -396,6 +397,11 @@ class Context:
     def cc_residual_dev(self, dT1, dT2, dR1, dR2, method=ECG_CCSD, precision=ECG_FP64):
         """Device-pointer variant: arguments are integer device addresses."""
         self._chk(self._lib.ecg_cc_residual_dev(self._h, method, precision, dT1, dT2, dR1, dR2))
+
+    def cc_residual_dev_noalloc(self, dT1, dT2, dR1, dR2, method=ECG_CCSD, precision=ECG_FP64):
+        """Allocation-free device-pointer variant (FP64; scratch in the library's static
+        device array of ECG_NOALLOC_SCRATCH_BYTES per GPU)."""
+        self._chk(self._lib.ecg_cc_residual_dev_noalloc(self._h, method, precision, dT1, dT2, dR1, dR2))
 
     def cc_solve(self, method=ECG_CCSD, precision=ECG_FP64, opts=None):
         opts = opts or default_opts()

@@ -163,6 +163,24 @@ int ecg_cc_residual(ecg_ctx* ctx, int method, int precision, const double* T1, c
 int ecg_cc_residual_dev(ecg_ctx* ctx, int method, int precision, const double* dT1, const double* dT2,
                         double* dR1, double* dR2);
 
+/* Allocation-free variant of ecg_cc_residual_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  FP64 only
+ * (precision must be ECG_FP64; products run on the unpacked FP64 MFMA kernel).
+ * Every contraction temporary, ladder integral block and split-K partial sum lives
+ * in one static device array of ECG_NOALLOC_SCRATCH_BYTES per GPU, held from the
+ * library's first use of that GPU until the process ends; the dressed factors were
+ * allocated by ecg_create.  A system whose residual does not fit the array is
+ * rejected with status -2 and an ecg_last_error message (at 256 MiB: o = 12,
+ * v = 108, naux = 360 fits; o = 20, v = 180 does not).  All calls on one GPU share
+ * the array: calls that overlap in time on one GPU (contexts driven from several
+ * threads) race; calls one after another are safe, since nothing in the array
+ * has to start at zero.  Results equal ecg_cc_residual_dev's in FP64 up to
+ * summation order. */
+#define ECG_NOALLOC_SCRATCH_BYTES (256u * 1024u * 1024u)
+int ecg_cc_residual_dev_noalloc(ecg_ctx* ctx, int method, int precision, const double* dT1, const double* dT2,
+                                double* dR1, double* dR2);
+
 /* Device-resident iterations of cc_iterations! (src/cc/cc.jl:2518-2648).
  * state_init: amplitudes from T1/T2 (host), or the MP2 guess when both are NULL;
  * resets DIIS.  state_step: one iteration (residual, Hylleraas energy, update,
