@@ -296,8 +296,10 @@ struct Planes {
   float* f = nullptr;
   double* d = nullptr;
   long ld = 0;
+  int K = 0;                 // data elements of a row (zeros from K to ld)
   // rows of src as in GemmEngine::pack_rows_bf16x3; 256 readable rows are kept after the last
-  void make(GemmEngine& g, const double* src, long rows, int d0, long s0, long s1, int K, long sk) {
+  void make(GemmEngine& g, const double* src, long rows, int d0, long s0, long s1, int K_, long sk) {
+    K = K_;
     ld = ((K + 31) / 32) * 32;
     const long rows_alloc = ((rows + 63) / 64) * 64 + 256;
     const long packed = ((rows + 63) / 64) * 64;
@@ -305,21 +307,21 @@ struct Planes {
       buf->ensure(sizeof(double) * rows_alloc * ld);
       d = static_cast<double*>(buf->p);
       ECG_HIP(hipMemsetAsync(d + packed * ld, 0, sizeof(double) * (rows_alloc - packed) * ld, g.stream));
-      g.pack_rows_f64(src, rows, d0, s0, s1, K, sk, (int)ld, d);
+      g.pack_rows_f64(src, rows, d0, s0, s1, K_, sk, (int)ld, d);
       return;
     }
     buf->ensure(sizeof(uint16_t) * 2 * rows_alloc * ld);   // 4 bytes per element either way
     if (f32) {
       f = static_cast<float*>(buf->p);
       ECG_HIP(hipMemsetAsync(f + packed * ld, 0, sizeof(float) * (rows_alloc - packed) * ld, g.stream));
-      g.pack_rows_f32(src, rows, d0, s0, s1, K, sk, (int)ld, f);
+      g.pack_rows_f32(src, rows, d0, s0, s1, K_, sk, (int)ld, f);
       return;
     }
     hi = static_cast<uint16_t*>(buf->p);
     lo = hi + rows_alloc * ld;
     ECG_HIP(hipMemsetAsync(hi + packed * ld, 0, sizeof(uint16_t) * (rows_alloc - packed) * ld, g.stream));
     ECG_HIP(hipMemsetAsync(lo + packed * ld, 0, sizeof(uint16_t) * (rows_alloc - packed) * ld, g.stream));
-    g.pack_rows_bf16x3(src, rows, d0, s0, s1, K, sk, (int)ld, hi, lo);
+    g.pack_rows_bf16x3(src, rows, d0, s0, s1, K_, sk, (int)ld, hi, lo);
   }
 };
 struct PackedInts { Planes W, Wt, Hn; };
@@ -373,6 +375,7 @@ void build_K_packed(GemmEngine& g, int v, int o, int i0, int nb, int j, int k, c
     s.lda = A.ld; s.a_z = aoff(i0 + 1) - a0;
     s.ldb = B.ld; s.b_z = boff(i0 + 1) - b0;
     s.K = (int)A.ld;
+    s.Kt = A.K;
   };
   PackedOp op;
   op.f32 = T.Ts.f32;
