@@ -71,9 +71,9 @@ struct PackedSeg {
   const uint16_t* Ah = nullptr; const uint16_t* Al = nullptr; long lda = 0;
   const uint16_t* Bh = nullptr; const uint16_t* Bl = nullptr; long ldb = 0;
   int K = 0;
-  // rows hold data in their first Kt elements and zeros up to K (0: Kt = K).  BF16 hi/lo
-  // products then run the segment over Kt rounded up to 8 elements (segments join inside a
-  // slab), FP64 products over Kt rounded up to 16 (their slab); FP32 products over K
+  // rows hold data in their first Kt elements and zeros up to K (0: Kt = K).  FP64 products
+  // then run the segment over Kt rounded up to 16 (their slab); BF16 hi/lo and FP32 ones
+  // over K
   int Kt = 0;
   long a_z = 0, b_z = 0;   // element offsets of A and B (both planes) per batch index
 };

@@ -184,8 +184,8 @@ __global__ __launch_bounds__(256) void pack_f64p_kernel(PackArgs p, int nrt) {
 
 struct X3Params {
   int M, N, kpad, kchunk, ksplit, tiles_m, tiles_n;
-  // K segments: k in [kb[s], kb[s+1]) reads row m of A at Ah[s] + m*lda[s] + (k - kb[s]); kb[s]
-  // are multiples of the slab (32) except in BF16 hi/lo products (8) and FP64 ones (16)
+  // K segments: slab k0 in [kb[s], kb[s+1]) reads row m of A at Ah[s] + m*lda[s] + (k0 - kb[s]);
+  // kb[s] are multiples of the slab (32; 16 in FP64 products)
   int nseg, kb[MAXSEG + 1];
   const u16* Ah[MAXSEG]; const u16* Al[MAXSEG]; long lda[MAXSEG];
   const u16* Bh[MAXSEG]; const u16* Bl[MAXSEG]; long ldb[MAXSEG];
@@ -460,62 +460,30 @@ __device__ __forceinline__ void dma16s(const void* base, unsigned off, lds_vptr 
 // A, 2 of B: six 1 KB loads per wave).  Lane l of a block loads row 16 c + l/4 into slot
 // l%4, i.e. chunk (l%4) ^ ((l/8)%4) of that row: a uniform block base plus a per-lane
 // byte offset that is the same for every block of an operand.  With several K segments
-// (MS) the planes and strides change along K (per-lane 64-bit addresses then cost fewer
-// scalar registers than uniform bases of every block); segments join at chunk
-// granularity, and in a slab that straddles two each lane takes the segment of its chunk.
+// (MS) the segment's planes and strides change along K; the per-lane 64-bit addresses
+// then cost fewer scalar registers than uniform bases of every block (which made the
+// compiler spill scalar registers inside the loop).
 template <bool MS>
 __device__ __forceinline__ void issue_bf16x3_slab(const X3Params& p, u32x4_t (&As)[2][XM][4],
                                                   u32x4_t (&Bs)[2][XN][4], int k0, long zb, int bm, int bn,
                                                   int wave, int lane) {
   const int lrow = lane >> 2, lch = (lane & 3) ^ ((lrow >> 1) & 3);
-  bool straddle = false;   // K segments start at multiples of 8: a slab may straddle two
-  if constexpr (MS) {
+  const SlabSrc g = slab_src<MS>(p, k0, zb);
+  const unsigned offa = (unsigned)(lrow * g.lda + 8 * lch) * 2u;
+  const unsigned offb = (unsigned)(lrow * g.ldb + 8 * lch) * 2u;
 #pragma unroll
-    for (int q = 1; q < MAXSEG; ++q) straddle = straddle || (q < p.nseg && k0 < p.kb[q] && k0 + XK > p.kb[q]);
+  for (int j = 0; j < 4; ++j) {
+    const int c = wave + 8 * j, pl = c >> 4, r0 = 16 * (c & 15);
+    const u16* pa = pl ? g.al : g.ah;
+    if constexpr (MS) dma16(pa + (size_t)(bm + r0 + lrow) * g.lda + g.kl + 8 * lch, (lds_vptr)&As[pl][r0][0]);
+    else dma16s(pa + (size_t)(bm + r0) * g.lda + g.kl, offa, (lds_vptr)&As[pl][r0][0]);
   }
-  if (straddle) {
-    // each lane takes the segment of its own chunk k0 + 8 lch
-    const int k = k0 + 8 * lch;
-    const u16 *ah = p.Ah[0], *al = p.Al[0], *bh = p.Bh[0], *bl = p.Bl[0];
-    long lda = p.lda[0], ldb = p.ldb[0], oa = p.az[0], ob = p.bz[0];
-    int kb = 0;
 #pragma unroll
-    for (int q = 1; q < MAXSEG; ++q)
-      if (q < p.nseg && k >= p.kb[q]) {
-        ah = p.Ah[q]; al = p.Al[q]; bh = p.Bh[q]; bl = p.Bl[q];
-        lda = p.lda[q]; ldb = p.ldb[q]; oa = p.az[q]; ob = p.bz[q]; kb = p.kb[q];
-      }
-    const long kl = (long)(k - kb);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = wave + 8 * j, pl = c >> 4, r0 = 16 * (c & 15);
-      const u16* pa = (pl ? al : ah) + zb * oa;
-      dma16(pa + (size_t)(bm + r0 + lrow) * lda + kl, (lds_vptr)&As[pl][r0][0]);
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int c = wave + 8 * j, pl = c >> 3, r0 = 16 * (c & 7);
-      const u16* pb = (pl ? bl : bh) + zb * ob;
-      dma16(pb + (size_t)(bn + r0 + lrow) * ldb + kl, (lds_vptr)&Bs[pl][r0][0]);
-    }
-  } else {
-    const SlabSrc g = slab_src<MS>(p, k0, zb);
-    const unsigned offa = (unsigned)(lrow * g.lda + 8 * lch) * 2u;
-    const unsigned offb = (unsigned)(lrow * g.ldb + 8 * lch) * 2u;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = wave + 8 * j, pl = c >> 4, r0 = 16 * (c & 15);
-      const u16* pa = pl ? g.al : g.ah;
-      if constexpr (MS) dma16(pa + (size_t)(bm + r0 + lrow) * g.lda + g.kl + 8 * lch, (lds_vptr)&As[pl][r0][0]);
-      else dma16s(pa + (size_t)(bm + r0) * g.lda + g.kl, offa, (lds_vptr)&As[pl][r0][0]);
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int c = wave + 8 * j, pl = c >> 3, r0 = 16 * (c & 7);
-      const u16* pb = pl ? g.bl : g.bh;
-      if constexpr (MS) dma16(pb + (size_t)(bn + r0 + lrow) * g.ldb + g.kl + 8 * lch, (lds_vptr)&Bs[pl][r0][0]);
-      else dma16s(pb + (size_t)(bn + r0) * g.ldb + g.kl, offb, (lds_vptr)&Bs[pl][r0][0]);
-    }
+  for (int j = 0; j < 2; ++j) {
+    const int c = wave + 8 * j, pl = c >> 3, r0 = 16 * (c & 7);
+    const u16* pb = pl ? g.bl : g.bh;
+    if constexpr (MS) dma16(pb + (size_t)(bn + r0 + lrow) * g.ldb + g.kl + 8 * lch, (lds_vptr)&Bs[pl][r0][0]);
+    else dma16s(pb + (size_t)(bn + r0) * g.ldb + g.kl, offb, (lds_vptr)&Bs[pl][r0][0]);
   }
 }
 
@@ -1089,11 +1057,13 @@ void GemmEngine::run_bf16x3_packed(const PackedOp& op) {
   x.M = op.M; x.N = op.N;
   x.tiles_m = (op.M + XM - 1) / XM; x.tiles_n = (op.N + XN - 1) / XN;
   x.nseg = op.nseg;
-  // segment lengths in the concatenated K: BF16 planes are read in 8-element chunks that
-  // may come from two segments within one slab, FP64 planes slab by slab (16), FP32 planes
-  // (permuted within 32-wide slabs) whole rows; the last slab's tail past the last segment
-  // must lie in that segment's zero padding, else every segment keeps its full row length
-  const int gran = op.f32 ? XK : (op.f64 ? 16 : 8), slabk = op.f64 ? 16 : XK;
+  // segment lengths in the concatenated K: FP64 products run slabs of 16, so an FP64
+  // segment ends at its data length rounded up to 16; BF16 hi/lo and FP32 products keep
+  // whole 32-wide slabs per segment (a BF16 slab that straddles two segments, addressed
+  // per lane, measured 11 % slower per product than the 10 % of K it saves); the last
+  // slab's tail past the last segment must lie in that segment's zero padding, else every
+  // segment keeps its full row length
+  const int gran = op.f64 ? 16 : XK, slabk = gran;
   int klen[MAXSEG];
   int ktot = 0;
   for (int s = 0; s < op.nseg; ++s) {
