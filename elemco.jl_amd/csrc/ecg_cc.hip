@@ -427,8 +427,13 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
     if (!dc) ops.contract_p(pr, 1.0, klcd, "klcd", tT2, "cdij", 1.0, tW, "klij");
     ops.contract_p(pr, 1.0, tW, "klij", tT2, "abkl", 0.0, tR2, "abij");
   }
-  // particle-particle ladder (kext): sum_Q L^[a,i,Q] R^[b,j,Q] + DF ladder
-  ops.contract_p(Prec::F64, 1.0, T3d(Ld.vo.d(), v, o, Q), "aiQ", T3d(Rh.vo.d(), v, o, Q), "bjQ", 1.0, tR2, "abij");
+  // particle-particle ladder (kext): sum_Q L^[a,i,Q] R^[b,j,Q] + DF ladder.  The three
+  // o^2 v^2 Q assemblies of the doubles (this one, d_voov and d_vovo below) run in FP32
+  // products when the residual's products are FP32 or BF16 hi/lo: the solver's FP64
+  // defect correction removes their rounding with that of the T products.  (In BF16 hi/lo
+  // they raised the corrected residual's floor: one more exact iteration per solve.)
+  const Prec pa = pr == Prec::F64 ? Prec::F64 : Prec::F32;
+  ops.contract_p(pa, 1.0, T3d(Ld.vo.d(), v, o, Q), "aiQ", T3d(Rh.vo.d(), v, o, Q), "bjQ", 1.0, tR2, "abij");
   double* X = take4();
   if (!dc) {
     // <kl|cd> T[a,d,k,j] T[c,b,i,l]   (cc.jl:1435-1439)
@@ -465,7 +470,7 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   {
     // Y[a,k,i,c] = d_voov[a,k,i,c] + 1/2 <kl|cd> T~[a,d,i,l];  R2r += Y T~[c,b,k,j]   (cc.jl:1462-1468)
     Tensor tY = T4d(X, v, o, o, v);
-    ops.contract_p(Prec::F64, 1.0, T3d(Ld.vo.d(), v, o, Q), "aiQ", T3d(Ru.ov.d(), o, v, Q), "kcQ", 0.0, tY, "akic");
+    ops.contract_p(pa, 1.0, T3d(Ld.vo.d(), v, o, Q), "aiQ", T3d(Ru.ov.d(), o, v, Q), "kcQ", 0.0, tY, "akic");
     ops.contract_p(pr, 0.5, klcd, "klcd", tTt, "adil", 1.0, tY, "akic");
     ops.contract_p(pr, 1.0, tY, "akic", tTt, "cbkj", 1.0, tR2r, "abij");
   }
@@ -482,7 +487,7 @@ void Context::residual(int method, Prec pr, const double* T1d, const double* T2d
   {
     // - d_vovo[a,k,c,i] T[c,b,k,j] - d_vovo[b,k,c,i] T[a,c,k,j],  d_vovo[a,k,c,i] = (ac|ki)^   (cc.jl:1476-1481)
     Tensor tD = T4d(X, v, o, v, o);
-    ops.contract_p(Prec::F64, 1.0, T3d(Ld.vv.d(), v, v, Q), "acQ", T3d(Rh.oo.d(), o, o, Q), "kiQ", 0.0, tD, "akci");
+    ops.contract_p(pa, 1.0, T3d(Ld.vv.d(), v, v, Q), "acQ", T3d(Rh.oo.d(), o, o, Q), "kiQ", 0.0, tD, "akci");
     ops.contract_p(pr, -1.0, tD, "akci", tT2, "cbkj", 1.0, tR2r, "abij");
     ops.contract_p(pr, -1.0, tD, "bkci", tT2, "ackj", 1.0, tR2r, "abij");
   }
