@@ -341,7 +341,9 @@ void Context::dev_launch(int method, const SolverOpts& o) {
     ECG_HIP(hipMemcpyAsync(d.info_host, d.scal.d() + S_INFO, sizeof(double) * 8, hipMemcpyDeviceToHost, stream));
     return;
   }
-  const std::vector<double> okey{(double)method, o.thr, o.conven, o.shifts, o.shiftp, o.resthr, (double)o.maxdiis};
+  // the split-K share is part of the key: it fixes the captured launches' grids and splits
+  const std::vector<double> okey{(double)method, o.thr,       o.conven, o.shifts, o.shiftp, o.resthr,
+                                 (double)o.maxdiis, (double)ops.gemm.cus};
   if (d.exec && (d.key != key_now() || d.okey != okey)) {
     (void)hipGraphExecDestroy(d.exec);
     d.exec = nullptr;

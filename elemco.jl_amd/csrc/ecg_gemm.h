@@ -154,6 +154,9 @@ struct GemmEngine {
   // product) instead of `ws`, no operand is packed into planes, and only FP64 products
   // run: the engine then allocates nothing (the allocation-free entry points).
   Arena* transient = nullptr;
+  // CUs a product may expect to have to itself when the split-K rules decide how far to
+  // split (256: the whole device; fewer while several contexts run products concurrently)
+  int cus = 256;
   hipEvent_t timing_begin();
   void timing_end(hipEvent_t e0);
 };

@@ -339,13 +339,13 @@ void GemmEngine::run_prec(const GemmOp& op, Prec pr) {
   p.a_s0 = op.a_s0; p.a_s1 = op.a_s1; p.a_z = op.a_z; p.a_d0 = op.a_d0;
   p.b_s0 = op.b_s0; p.b_s1 = op.b_s1; p.b_z = op.b_z; p.b_d0 = op.b_d0;
   p.alpha = op.alpha; p.beta = op.beta;
-  // split K when the output grid alone cannot fill 256 CUs (about 1024 workgroups, each
-  // with at least 8 slabs of K): a one-tile product such as x_ki (M = N = o, K = o v^2)
+  // split K when the output grid alone cannot fill the CUs (about 4 workgroups per CU,
+  // each with at least 8 slabs of K): a one-tile product such as x_ki (M = N = o, K = o v^2)
   // otherwise runs on a quarter of the CUs
   long tiles = (long)p.tiles_m * p.tiles_n * op.batch;
   int ksplit = 1;
-  if (tiles < 512 && p.K >= 4 * BKS * 8) {
-    ksplit = (int)std::min<long>((1024 + tiles - 1) / tiles, p.K / (BKS * 8));
+  if (tiles < 2 * cus && p.K >= 4 * BKS * 8) {
+    ksplit = (int)std::min<long>((4 * cus + tiles - 1) / tiles, p.K / (BKS * 8));
     ksplit = std::max(1, std::min(ksplit, 1024));
   }
   long slab = (long)op.M * op.N;

@@ -960,17 +960,18 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
     x.Ah[s] = Ah; x.Al[s] = Al; x.Bh[s] = Bh; x.Bl[s] = Bl; x.lda[s] = kpad; x.ldb[s] = kpad;
   }
   x.alpha = op.alpha; x.beta = op.beta;
-  // split K when the tiles alone cannot occupy the 256 CUs (one workgroup per CU); for
-  // deep products with a few rounds of tiles, split as far as it cuts the partly filled
-  // last round (rounds per unit of work ceil(tiles ks / 256) / ks, at least 4096 of K per
-  // split, at least 5 % gained): the ladder's N = o^2 contractions have K = v^2
+  // split K when the tiles alone cannot occupy the CUs (one workgroup per CU; `cus` of them,
+  // 256 unless several contexts share the device); for deep products with a few rounds of
+  // tiles, split as far as it cuts the partly filled last round (rounds per unit of work
+  // ceil(tiles ks / cus) / ks, at least 4096 of K per split, at least 5 % gained): the
+  // ladder's N = o^2 contractions have K = v^2
   const long tiles = (long)tiles_m * tiles_n;
   int ksplit = 1;
-  if (tiles < 256 && kpad >= 8 * XK) {
-    ksplit = (int)std::min<long>((512 + tiles - 1) / tiles, kpad / (4 * XK));
+  if (tiles < cus && kpad >= 8 * XK) {
+    ksplit = (int)std::min<long>((2 * cus + tiles - 1) / tiles, kpad / (4 * XK));
     ksplit = std::max(1, std::min(ksplit, 64));
-  } else if (tiles < 16 * 256 && kpad >= 2 * 4096) {
-    auto rounds = [&](int ks) { return (double)((tiles * ks + 255) / 256) / ks; };
+  } else if (tiles < 16 * cus && kpad >= 2 * 4096) {
+    auto rounds = [&](int ks) { return (double)((tiles * ks + cus - 1) / cus) / ks; };
     double best = rounds(1);
     for (int ks = 2; ks <= 8 && kpad / ks >= 4096; ++ks)
       if (rounds(ks) < 0.95 * best) { best = rounds(ks); ksplit = ks; }

@@ -70,3 +70,21 @@ def test_batch_with_graph_replay_matches_eager():
     e1, it1, c1 = E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, E.default_opts(graphs=1, **opts))
     assert c0.all() and c1.all() and (it0 == it1).all()
     assert float(np.max(np.abs(e1[:, 0] - e0[:, 0]))) < 1e-12
+
+
+def test_batch_leaves_single_solves_unchanged():
+    """During a batch every product splits K for a quarter of the device (the systems share
+    the CUs); afterwards each context's own solves split as before, so a single solve gives
+    bit-identical energies before and after a batch (eager and with graph replay)."""
+    _, ecs = _systems()
+    ctxs = [ec.ctx for ec in ecs[-2:]]
+    opts = E.default_opts(maxit=100, thr=1e-12)
+    gopts = E.default_opts(maxit=100, thr=1e-12, graphs=1)
+    before = [c.cc_solve(E.ECG_CCSD, E.ECG_FP64, opts)[0].copy() for c in ctxs]
+    E.cc_solve_batch(ctxs, E.ECG_CCSD, E.ECG_FP64, gopts)
+    after = [c.cc_solve(E.ECG_CCSD, E.ECG_FP64, opts)[0].copy() for c in ctxs]
+    for b, a in zip(before, after):
+        assert np.array_equal(a, b)
+    g = [c.cc_solve(E.ECG_CCSD, E.ECG_FP64, gopts)[0].copy() for c in ctxs]
+    for b, a in zip(before, g):
+        assert abs(float(a[0]) - float(b[0])) < 1e-12
