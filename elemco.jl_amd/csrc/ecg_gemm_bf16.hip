@@ -993,14 +993,23 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
   ECG_CHECK(ksplit <= 65535, "gemm: split-K grid limit");
   hipEvent_t e0 = timing_begin();
   const dim3 grid((unsigned)tiles, 1, (unsigned)ksplit);
-  if (kind == 2)
-    hipLaunchKernelGGL((gemm_f64p_dma_kernel<false>), grid, dim3(512), 0, stream, x);
-  else if (kind == 1)
-    hipLaunchKernelGGL((gemm_f32p_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
-  else if (bf16_kernel == 1)
-    hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
-  else
-    hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), grid, dim3(512), 0, stream, x);
+  // transposed accumulators (lanes along m) when m is the output's unit stride, as in the
+  // split-K partial slabs: a store instruction then writes 4 whole 128-byte rows of C
+  // instead of touching 16
+  const bool to = x.cplain || (x.c_ms0 == 1 && x.c_ns0 != 1);
+  if (kind == 2) {
+    if (to) hipLaunchKernelGGL((gemm_f64p_dma_kernel<true>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_f64p_dma_kernel<false>), grid, dim3(512), 0, stream, x);
+  } else if (kind == 1) {
+    if (to) hipLaunchKernelGGL((gemm_f32p_dma_kernel<true, false>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_f32p_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
+  } else if (bf16_kernel == 1) {
+    if (to) hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<true, false>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<false, false>), grid, dim3(512), 0, stream, x);
+  } else {
+    if (to) hipLaunchKernelGGL((gemm_bf16x3_kernel<true, false>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_bf16x3_kernel<false, false>), grid, dim3(512), 0, stream, x);
+  }
   ECG_HIP(hipGetLastError());
   timing_end(e0);
   if (ksplit > 1) reduce_slabs(op, ksplit, ws.d());
