@@ -326,8 +326,96 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
   df_ladder_half(ops, nv, no, nq, pr, Lf, Rf, T2x, R2r);
 }
 
+// The ladder with the V blocks written straight into the packed planes of the contraction's
+// A operand (GemmEngine::run_planes with a PlaneOut): V never exists in FP64 and is not read
+// back for packing.  The V product runs on the factors transposed to [c, a, Q] (one pass
+// each per residual), so its rows are m = c + v a' and its columns n = d + v b', and its
+// stores walk c: consecutive positions c + v d of plane rows a' + na b', the layout the
+// contraction's own packing of V[a',b',c,d] would write.  Every product is the unfused
+// one's, so the results are the same.  Needs the packed-plane kernels and v^2 a multiple
+// of 32 (no K padding to clear); returns false otherwise.
+static bool df_ladder_half_planes(TensorOps& ops, long nv, long no, int nq, Prec pr, const double* Lf,
+                                  const double* Rf, const double* T2x, double* R2r) {
+  GemmEngine& G = ops.gemm;
+  const long v = nv, vv = v * v, oo = no * no;
+  if (G.transient || vv % 32 != 0 || oo < 128 || v < 64) return false;
+  if ((pr == Prec::F64 && G.f64_kernel != 1) || (pr == Prec::F32 && G.f32_kernel != 1)) return false;
+  const int kind = pr == Prec::BF16X3 ? 0 : (pr == Prec::F32 ? 1 : 2);
+  const long esz = kind == 2 ? 8 : 4;   // bytes per plane element (BF16: hi + lo)
+  const long kpad = vv;
+  size_t mark = ops.arena.mark();
+  const int p_ca[3] = {1, 0, 2};
+  double* Lt = ops.arena.take(vv * nq);
+  {
+    Tensor in(const_cast<double*>(Lf), {v, v, (long)nq}), out(Lt, {v, v, (long)nq});
+    ops.permute(in, p_ca, out, 1.0, 0.0);
+  }
+  double* Rt = Lt;
+  if (Rf != Lf) {
+    Rt = ops.arena.take(vv * nq);
+    Tensor in(const_cast<double*>(Rf), {v, v, (long)nq}), out(Rt, {v, v, (long)nq});
+    ops.permute(in, p_ca, out, 1.0, 0.0);
+  }
+  // planes of ab a-rows: rows a' + na b' (b' < v - a0), plus 256 rows that the contraction's
+  // last tile reads (their results are discarded).  Each block's diagonal part computes both
+  // (a',b') and (b',a'), so the ladder's work grows with the block (v^2 (v + ab) / 2 rows of
+  // V in all), against smaller products for small blocks: at most 32 a-rows (the c2
+  // iteration, o=40 v=360: FP64 1752 ms with 32 rows, 1779 with 24, 1786 with 40, 1794 with
+  // the 53 that the arena allows for FP64 planes; BF16 hi/lo 406, 410, 412, 422 ms), and no
+  // more than FP64 planes of 80 % of the free arena
+  const long free_b = (long)ops.arena.free_bytes();
+  const long ab = std::max<long>(
+      1, std::min<long>(std::min<long>(v, 32), ((long)(free_b * 0.8) - 256 * kpad * 8) / (v * kpad * 8)));
+  const long nrv = ab * v + 256;
+  char* vp = reinterpret_cast<char*>(ops.arena.take((nrv * kpad * esz + 7) / 8));
+  G.drop_kept_planes();
+  for (long a0 = 0; a0 < v; a0 += ab) {
+    const long na = std::min<long>(ab, v - a0);
+    const long nb = v - a0;
+    // V[a',b',c,d] = sum_Q L^[a0+a',c,Q] R^[a0+b',d,Q]   (m = c + v a', n = d + v b', k = Q)
+    GemmOp g;
+    g.M = (int)(na * v); g.N = (int)(nb * v); g.K = nq;
+    g.A = Lt + v * a0; g.a_s0 = 1; g.a_k = vv;
+    g.B = Rt + v * a0; g.b_s0 = 1; g.b_k = vv;
+    g.alpha = 1.0; g.beta = 0.0;
+    PlaneOut po;
+    po.hi = vp;
+    po.lo = kind == 0 ? static_cast<void*>(reinterpret_cast<uint16_t*>(vp) + nrv * kpad) : nullptr;
+    po.ld = kpad; po.dm = (int)v; po.dn = (int)v; po.ra = 1; po.rb = na; po.kn = v;
+    G.run_planes(g, kind, &po);
+    // R2r[a0+a',a0+b',i,j] += w sum_{cd} V[a',b',c,d] T[c,d,i,j] on plane rows a' + na b'
+    for (int part = 0; part < 2; ++part) {
+      const long b0 = part == 0 ? 0 : na;           // diagonal block, then b' >= na
+      const long nbp = part == 0 ? na : nb - na;
+      if (nbp <= 0) continue;
+      const long r0 = na * b0;
+      GemmOp h;
+      h.M = (int)(na * nbp); h.N = (int)oo; h.K = (int)vv;
+      h.a_planes_ld = kpad;
+      if (kind == 2) {
+        h.a_planes = reinterpret_cast<const double*>(vp) + r0 * kpad;
+      } else if (kind == 1) {
+        h.a_planes = reinterpret_cast<const float*>(vp) + r0 * kpad;
+      } else {
+        h.a_planes = reinterpret_cast<const uint16_t*>(vp) + r0 * kpad;
+        h.a_planes_lo = reinterpret_cast<const uint16_t*>(vp) + nrv * kpad + r0 * kpad;
+      }
+      h.B = T2x; h.b_s0 = 1; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = oo;
+      h.C = R2r + a0 + (a0 + b0) * v; h.c_ms0 = 1; h.c_md0 = (int)na; h.c_ms1 = v;
+      h.c_ns0 = vv; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
+      h.alpha = part == 0 ? 0.5 : 1.0; h.beta = 1.0;
+      h.b_keep = true;
+      G.run_planes(h, kind);
+    }
+  }
+  G.drop_kept_planes();
+  ops.arena.release_to(mark);
+  return true;
+}
+
 void df_ladder_half(TensorOps& ops, long nv, long no, int nq, Prec pr, const double* Lf, const double* Rf,
                     const double* T2x, double* R2r) {
+  if (df_ladder_half_planes(ops, nv, no, nq, pr, Lf, Rf, T2x, R2r)) return;
   size_t mark = ops.arena.mark();
   size_t free_b = ops.arena.free_bytes();
   const long v = nv;

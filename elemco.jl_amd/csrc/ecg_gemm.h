@@ -42,6 +42,13 @@ struct GemmOp {
   // BF16 hi/lo products: B's data does not change until GemmEngine::drop_kept_planes(), so
   // its packed planes are kept and reused by the next product with the same B operand
   bool b_keep = false;
+  // Packed-plane products: A already packed in the layout the product's own packing
+  // writes (BF16 hi plane a_planes and lo plane a_planes_lo, or the FP32 / FP64 plane),
+  // rows of a_planes_ld = K padded to 32 elements, readable up to M rounded up to 256
+  // rows; A, a_s0/a_s1/a_d0 and a_k are then unused and A is not packed
+  const void* a_planes = nullptr;
+  const void* a_planes_lo = nullptr;
+  long a_planes_ld = 0;
 
   // Plain column-major helpers: A is M x K (lda), or K x M when transposed.
   GemmOp& a_colmajor(const double* p, long lda, bool trans) {
@@ -102,6 +109,19 @@ struct PackedOp {
   bool f64 = false;
 };
 
+// Output of a packed-plane product written into packed planes of a later product's A
+// operand instead of C: element (m, n), m = c + dm a' (c < dm), n = d + dn b' (d < dn),
+// goes to plane row a' ra + b' rb, K position c + kn d, in the layout of the product
+// kind's packing (BF16 hi and lo planes, the FP32 or the FP64 plane; rows of ld elements).
+// The product stores with lanes along m, i.e. along consecutive positions of a row.
+struct PlaneOut {
+  void* hi = nullptr;   // BF16 hi plane, or the FP32 / FP64 plane
+  void* lo = nullptr;   // BF16 lo plane
+  long ld = 0;
+  int dm = 1, dn = 1;
+  long ra = 0, rb = 0, kn = 0;
+};
+
 // Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,
 // FP32 accumulation), or BF16 hi/lo split (three BF16 MFMA products per pair,
 // FP32 accumulation).  Results are always returned in FP64.
@@ -131,7 +151,9 @@ struct GemmEngine {
   void run_f32p(const GemmOp& op);
   // FP64 products on FP64 planes packed on the fly (K slabs of 16)
   void run_f64p(const GemmOp& op);
-  void run_planes(const GemmOp& op, int kind);   // kind 0 BF16 hi/lo, 1 FP32, 2 FP64 planes
+  // kind 0 BF16 hi/lo, 1 FP32, 2 FP64 planes; with po the result goes to planes of the
+  // same kind (beta 0, K not split)
+  void run_planes(const GemmOp& op, int kind, const PlaneOut* po = nullptr);
   int f32_kernel = 1;   // large FP32 products: 0 generic FP32 MFMA kernel, 1 packed FP32 planes
   int f64_kernel = 1;   // large FP64 products: 0 generic FP64 MFMA kernel, 1 packed FP64 planes
   // packed B planes kept for products with GemmOp::b_keep (one operand, identified by its

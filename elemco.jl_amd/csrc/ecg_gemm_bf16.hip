@@ -194,7 +194,34 @@ struct X3Params {
   double alpha, beta;
   int cplain; long ldc_ws;
   int box_class, box_v;   // boxed output (see PackedOp)
+  // plane output (see PlaneOut): 0 none, 1 BF16 hi/lo planes, 2 FP32 plane, 3 FP64 plane
+  int pout_kind, pout_dm, pout_dn;
+  long pout_ra, pout_rb, pout_kn, pout_ld;
+  void* pout_hi;
+  void* pout_lo;
 };
+
+// Store x as element (m, n) of a plane output (PlaneOut): m = c + dm a', n = d + dn b' goes to
+// row a' ra + b' rb, position k = c + kn d, in the layout pack_*_kernel writes for the kind:
+// BF16 hi and lo planes unpermuted, with the same split of x as the packing; FP32 32-wide
+// slabs with position 8 (k % 4) + (k % 32) / 4 holding k; FP64 16-wide slabs with
+// 4 (k % 4) + (k % 16) / 4.
+__device__ __forceinline__ void plane_put(const X3Params& p, int a, int c, int b, int d, double x) {
+  const long row = (long)a * p.pout_ra + (long)b * p.pout_rb;
+  const long k = (long)c + p.pout_kn * (long)d;
+  if (p.pout_kind == 3) {
+    const long pos = (k & ~15L) | ((k & 3) << 2) | ((k >> 2) & 3);
+    reinterpret_cast<double*>(p.pout_hi)[row * p.pout_ld + pos] = x;
+  } else if (p.pout_kind == 2) {
+    const long pos = (k & ~31L) | ((k & 3) << 3) | ((k >> 2) & 7);
+    reinterpret_cast<float*>(p.pout_hi)[row * p.pout_ld + pos] = (float)x;
+  } else {
+    const u16 h = bf16_bits((float)x);
+    const u16 l = bf16_bits((float)(x - (double)bf16_val(h)));
+    reinterpret_cast<u16*>(p.pout_hi)[row * p.pout_ld + k] = h;
+    reinterpret_cast<u16*>(p.pout_lo)[row * p.pout_ld + k] = l;
+  }
+}
 
 // Offset of element (m, n) of a boxed output (PackedOp::box_class).
 __device__ __forceinline__ long box_off(int m, int n, int cls, int bv) {
@@ -216,6 +243,24 @@ __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 
 template <bool TO, typename A4 = f4_t, bool L64 = false>
 __device__ __forceinline__ void x3_epilogue(const X3Params& p, const A4 (&acc)[4][4], int bm, int bn, int wm,
                                             int wn, int li, int lk, int kz, long zb) {
+  if (p.pout_kind) {
+    // plane output (the host launches it with TO, lanes along m): (a', c) once per row
+    // block and (b', d) once per column, 32-bit divisions
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = L64 ? lk + 4 * r : 4 * lk + r;
+          const int n = bn + wn * 64 + j * 16 + (TO ? q : li);
+          const int m = bm + wm * 64 + i * 16 + (TO ? li : q);
+          const int a = m / p.pout_dm, b = n / p.pout_dn;
+          if (n < p.N && m < p.M)
+            plane_put(p, a, m - a * p.pout_dm, b, n - b * p.pout_dn, p.alpha * (double)acc[i][j][r]);
+        }
+    return;
+  }
   double* C = p.C + (p.cplain ? (long)kz * p.c_z : 0) + zb * p.cbz;
   float* Cf = p.Cf ? p.Cf + zb * p.cbz : nullptr;
 #pragma unroll
@@ -873,7 +918,7 @@ void GemmEngine::run_f64p(const GemmOp& op) { run_planes(op, 2); }
 
 // Operands packed on the fly (kind 0: BF16 hi/lo planes; 1: one FP32 plane; 2: one FP64
 // plane), then the packed-plane kernel; split K when the tiles alone cannot fill the GPU.
-void GemmEngine::run_planes(const GemmOp& op, int kind) {
+void GemmEngine::run_planes(const GemmOp& op, int kind, const PlaneOut* po) {
   ECG_CHECK(op.batch == 1, "gemm: batched packed products are not supported");
   ECG_CHECK(!transient, "gemm: no packed products into a caller's scratch region");
   GemmSeg one;
@@ -887,7 +932,7 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
   long ktot = 0;
   pa.kp[0] = pb.kp[0] = 0;
   for (int s = 0; s < nseg; ++s) {
-    ECG_CHECK(sg[s].K > 0 && sg[s].A && sg[s].B, "gemm: empty K segment");
+    ECG_CHECK(sg[s].K > 0 && (sg[s].A || op.a_planes) && sg[s].B, "gemm: empty K segment");
     pa.src[s] = sg[s].A; pa.sk[s] = sg[s].a_k; pa.sK[s] = sg[s].K;
     pb.src[s] = sg[s].B; pb.sk[s] = sg[s].b_k; pb.sK[s] = sg[s].K;
     kpad += ((sg[s].K + XK - 1) / XK) * XK;
@@ -904,6 +949,13 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
   const long Mp = (long)tiles_m * XM, Np = (long)tiles_n * XN;
   // bytes per element of a packed operand (two BF16 planes or one FP32 plane: 4; FP64: 8)
   const size_t esz = kind == 2 ? 8 : 4;
+  // A given as planes: no A packing, no room for it in pk
+  const bool pack_a = op.a_planes == nullptr;
+  ECG_CHECK(pack_a || (nseg == 1 && op.a_planes_ld == kpad && (kind != 0 || op.a_planes_lo)),
+            "gemm: packed A planes need one K segment, rows of K padded to 32 and (BF16) both planes");
+  ECG_CHECK(!po || (op.beta == 0.0 && po->hi && (kind != 0 || po->lo) && po->dm > 0 && po->dn > 0),
+            "gemm: plane output needs beta = 0 and its planes");
+  const long Ma = pack_a ? Mp : 0;
   char *a_base, *b_base;
   bool pack_b = true;
   if (op.b_keep) {
@@ -921,16 +973,17 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
       pk_keep.ensure(esz * (size_t)Np * kpad);
       keep_key = key;
     }
-    pk.ensure(esz * (size_t)Mp * kpad);
+    if (pack_a) pk.ensure(esz * (size_t)Mp * kpad);
     a_base = static_cast<char*>(pk.p);
     b_base = static_cast<char*>(pk_keep.p);
   } else {
-    pk.ensure(esz * (size_t)(Mp + Np) * kpad);
+    pk.ensure(esz * (size_t)(Ma + Np) * kpad);
     a_base = static_cast<char*>(pk.p);
-    b_base = a_base + esz * (size_t)Mp * kpad;
+    b_base = a_base + esz * (size_t)Ma * kpad;
   }
-  u16* Ah = reinterpret_cast<u16*>(a_base);
-  u16* Al = Ah + (size_t)Mp * kpad;   // BF16 lo plane (kind 0)
+  u16* Ah = pack_a ? reinterpret_cast<u16*>(a_base) : const_cast<u16*>(static_cast<const u16*>(op.a_planes));
+  u16* Al = pack_a ? Ah + (size_t)Mp * kpad   // BF16 lo plane (kind 0)
+                   : const_cast<u16*>(static_cast<const u16*>(op.a_planes_lo));
   u16* Bh = reinterpret_cast<u16*>(b_base);
   u16* Bl = Bh + (size_t)Np * kpad;
   // an FP32 or FP64 plane starts at the hi plane's address, rows of kpad elements
@@ -941,13 +994,13 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
   const long nba = (Mp / PR) * (long)(kpad / XK), nbb = (Np / PR) * (long)(kpad / XK);
   ECG_CHECK(nba < (1L << 31) && nbb < (1L << 31), "gemm: operand too large for the packing grid");
   if (kind == 2) {
-    hipLaunchKernelGGL(pack_f64p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+    if (pack_a) hipLaunchKernelGGL(pack_f64p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
     if (pack_b) hipLaunchKernelGGL(pack_f64p_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
   } else if (kind == 1) {
-    hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+    if (pack_a) hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
     if (pack_b) hipLaunchKernelGGL(pack_f32p_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
   } else {
-    hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
+    if (pack_a) hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nba), dim3(256), 0, stream, pa, (int)(Mp / PR));
     if (pack_b) hipLaunchKernelGGL(pack_bf16x3_kernel, dim3((unsigned)nbb), dim3(256), 0, stream, pb, (int)(Np / PR));
   }
   ECG_HIP(hipGetLastError());
@@ -967,7 +1020,9 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
   // ladder's N = o^2 contractions have K = v^2
   const long tiles = (long)tiles_m * tiles_n;
   int ksplit = 1;
-  if (tiles < cus && kpad >= 8 * XK) {
+  if (po) {
+    // plane output: no partial sums
+  } else if (tiles < cus && kpad >= 8 * XK) {
     ksplit = (int)std::min<long>((2 * cus + tiles - 1) / tiles, kpad / (4 * XK));
     ksplit = std::max(1, std::min(ksplit, 64));
   } else if (tiles < 16 * cus && kpad >= 2 * 4096) {
@@ -996,7 +1051,13 @@ void GemmEngine::run_planes(const GemmOp& op, int kind) {
   // transposed accumulators (lanes along m) when m is the output's unit stride, as in the
   // split-K partial slabs: a store instruction then writes 4 whole 128-byte rows of C
   // instead of touching 16
-  const bool to = x.cplain || (x.c_ms0 == 1 && x.c_ns0 != 1);
+  const bool to = po || x.cplain || (x.c_ms0 == 1 && x.c_ns0 != 1);
+  if (po) {
+    x.pout_kind = kind == 2 ? 3 : (kind == 1 ? 2 : 1);
+    x.pout_dm = po->dm; x.pout_dn = po->dn;
+    x.pout_ra = po->ra; x.pout_rb = po->rb; x.pout_kn = po->kn; x.pout_ld = po->ld;
+    x.pout_hi = po->hi; x.pout_lo = po->lo;
+  }
   if (kind == 2) {
     if (to) hipLaunchKernelGGL((gemm_f64p_dma_kernel<true>), grid, dim3(512), 0, stream, x);
     else hipLaunchKernelGGL((gemm_f64p_dma_kernel<false>), grid, dim3(512), 0, stream, x);

@@ -113,7 +113,10 @@ def test_ucc_closed_shell_limit_equals_ccsd(um, cm):
 def test_ucc_triples_match_oracle_and_reference():
     """(T) of UCCSD (ecg_upert_triples) on the H2O triplet: equals the oracle's spin-orbital (T)
     at the same converged amplitudes (1e-10 relative, E(T) and E[T]) and gives the reference's
-    UCCSD(T) energy of test/h2o_triplet.jl:8 (1e-6)."""
+    UCCSD(T) energy of test/h2o_triplet.jl:8 (1e-6).  The (T) of the solve's device-resident
+    amplitudes and of the same amplitudes passed back as blocks agree to 1e-12 relative: the
+    blocks are rebuilt exactly antisymmetric with the spin-forbidden elements zero, which the
+    solver's spin-orbital T2 holds only to rounding."""
     S, fd = _oracle("H2O.FCIDUMP.gz", 2)
     ctx = E.UContext.from_fcidump(os.path.join(GOLDEN, "H2O.FCIDUMP.gz"), ms2=2)
     try:
@@ -121,7 +124,7 @@ def test_ucc_triples_match_oracle_and_reference():
         assert conv
         et, etb = ctx.pert_triples()
         et2, etb2 = ctx.pert_triples(T)
-        assert abs(et2 - et) < 1e-14 and abs(etb2 - etb) < 1e-14
+        assert abs(et2 - et) < 1e-12 * abs(et) and abs(etb2 - etb) < 1e-12 * abs(etb)
         eto, etbo = U.calc_pertT_unrestricted(S, *S.from_blocks(*T))
         print(f"\nU(T): {et:.12f} oracle {eto:.12f}; UCCSD(T) {e[0] + et:.12f} reference -0.295388574918", flush=True)
         assert abs(et - eto) < 1e-10 * abs(eto) and abs(etb - etbo) < 1e-10 * abs(etbo)
