@@ -326,23 +326,56 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
   df_ladder_half(ops, nv, no, nq, pr, Lf, Rf, T2x, R2r);
 }
 
+#ifndef LADDER_MAX_ROWS
+#define LADDER_MAX_ROWS 64
+#endif
+
+// R2r[a0+a',a0+b',ij] += w Y[t,ij] for the triangle rows t = b'(b'+1)/2 + a' (a' <= b' < na),
+// w = 1 off the diagonal and 1/2 on it (the later symmetrisation adds the transposed half)
+__global__ void ladder_tri_scatter_kernel(int na, long v, long a0, long ntri, long nij, const double* __restrict__ Y,
+                                          double* R2r) {
+  const long n = ntri * nij;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long t = e % ntri, ij = e / ntri;
+    int b = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((long)b * (b + 1) / 2 > t) --b;
+    while ((long)(b + 1) * (b + 2) / 2 <= t) ++b;
+    const int a = (int)(t - (long)b * (b + 1) / 2);
+    R2r[(a0 + a) + v * (a0 + b) + v * v * ij] += (a == b ? 0.5 : 1.0) * Y[e];
+  }
+}
+
+// Zero the last 32-element slab of every row of a plane (rows of ld elements of esz bytes):
+// the K padding past v^2, which no V product writes
+__global__ void zero_plane_tail_kernel(char* plane, long rows, long ld, int esz) {
+  const long nw = (long)esz * 32 / 8, n = rows * nw;   // 8-byte words per row tail
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long r = e / nw, w = e - r * nw;
+    reinterpret_cast<unsigned long long*>(plane + (r * ld + ld - 32) * esz)[w] = 0ull;
+  }
+}
+
 // The ladder with the V blocks written straight into the packed planes of the contraction's
 // A operand (GemmEngine::run_planes with a PlaneOut): V never exists in FP64 and is not read
-// back for packing.  The V product runs on the factors transposed to [c, a, Q] (one pass
-// each per residual), so its rows are m = c + v a' and its columns n = d + v b', and its
-// stores walk c: consecutive positions c + v d of plane rows a' + na b', the layout the
-// contraction's own packing of V[a',b',c,d] would write.  Every product is the unfused
-// one's, so the results are the same.  Needs the packed-plane kernels and v^2 a multiple
-// of 32 (no K padding to clear); returns false otherwise.
+// back for packing.  The V products run on the factors transposed to [c, a, Q] (one pass
+// each per residual), so their rows are m = c + v a' and their columns n = d + v b', and
+// their stores walk c: consecutive positions c + v d of a plane row, the layout the
+// contraction's own packing of V[a',b',c,d] would write.  Per block of na a-rows:
+//  - b' >= na: plane rows a' + na (b' - na), contracted straight into R2r;
+//  - b' < na (the diagonal block): only a' <= b', at rows b'(b'+1)/2 + a' (tiles with none
+//    are skipped), contracted into a small temporary and added to R2r with weight 1, or
+//    1/2 for a' = b' (the symmetrisation supplies the (b',a') half),
+// so every pair a <= b is formed once.  K (= v^2) is padded to a multiple of 32 with zeros.
+// Needs the packed-plane kernels, o^2 >= 128 and v >= 64; returns false otherwise.
 static bool df_ladder_half_planes(TensorOps& ops, long nv, long no, int nq, Prec pr, const double* Lf,
                                   const double* Rf, const double* T2x, double* R2r) {
   GemmEngine& G = ops.gemm;
   const long v = nv, vv = v * v, oo = no * no;
-  if (G.transient || vv % 32 != 0 || oo < 128 || v < 64) return false;
+  if (G.transient || oo < 128 || v < 64) return false;
   if ((pr == Prec::F64 && G.f64_kernel != 1) || (pr == Prec::F32 && G.f32_kernel != 1)) return false;
   const int kind = pr == Prec::BF16X3 ? 0 : (pr == Prec::F32 ? 1 : 2);
   const long esz = kind == 2 ? 8 : 4;   // bytes per plane element (BF16: hi + lo)
-  const long kpad = vv;
+  const long kpad = ((vv + 31) / 32) * 32;
   size_t mark = ops.arena.mark();
   const int p_ca[3] = {1, 0, 2};
   double* Lt = ops.arena.take(vv * nq);
@@ -356,57 +389,87 @@ static bool df_ladder_half_planes(TensorOps& ops, long nv, long no, int nq, Prec
     Tensor in(const_cast<double*>(Rf), {v, v, (long)nq}), out(Rt, {v, v, (long)nq});
     ops.permute(in, p_ca, out, 1.0, 0.0);
   }
-  // planes of ab a-rows: rows a' + na b' (b' < v - a0), plus 256 rows that the contraction's
-  // last tile reads (their results are discarded).  Each block's diagonal part computes both
-  // (a',b') and (b',a'), so the ladder's work grows with the block (v^2 (v + ab) / 2 rows of
-  // V in all), against smaller products for small blocks: at most 32 a-rows (the c2
-  // iteration, o=40 v=360: FP64 1752 ms with 32 rows, 1779 with 24, 1786 with 40, 1794 with
-  // the 53 that the arena allows for FP64 planes; BF16 hi/lo 406, 410, 412, 422 ms), and no
-  // more than FP64 planes of 80 % of the free arena
-  const long free_b = (long)ops.arena.free_bytes();
-  const long ab = std::max<long>(
-      1, std::min<long>(std::min<long>(v, 32), ((long)(free_b * 0.8) - 256 * kpad * 8) / (v * kpad * 8)));
-  const long nrv = ab * v + 256;
+  // planes of a block of ab a-rows: the triangle (ab (ab+1)/2 rows), then the ab (v - a0 - ab)
+  // off-diagonal rows, plus 256 rows that a contraction's last tile reads (results
+  // discarded); the temporary of the triangle's contraction
+  const long free_b = (long)ops.arena.free_bytes() - 256 * kpad * esz;
+  long ab = std::min<long>(v, LADDER_MAX_ROWS);
+  auto need = [&](long r) { return (r * v + r * (r + 1) / 2) * kpad * esz + 8 * (r * (r + 1) / 2) * oo; };
+  while (ab > 1 && need(ab) > (long)(0.8 * (double)free_b)) --ab;
+  const long nrv = ab * v + ab * (ab + 1) / 2 + 256;
   char* vp = reinterpret_cast<char*>(ops.arena.take((nrv * kpad * esz + 7) / 8));
+  double* ytri = ops.arena.take(ab * (ab + 1) / 2 * oo);
+  if (kpad != vv) {
+    // BF16: the hi and lo planes are one run of 2 nrv rows of 2-byte elements
+    const long rows = kind == 0 ? 2 * nrv : nrv;
+    const int pe = kind == 0 ? 2 : (int)esz;
+    hipLaunchKernelGGL(zero_plane_tail_kernel, dim3((unsigned)std::min<long>((rows * pe * 4 + 255) / 256, 65535)),
+                       dim3(256), 0, ops.stream, vp, rows, kpad, pe);
+    ECG_HIP(hipGetLastError());
+  }
+  auto planes_at = [&](GemmOp& h, long r0) {
+    h.a_planes_ld = kpad;
+    if (kind == 2) {
+      h.a_planes = reinterpret_cast<const double*>(vp) + r0 * kpad;
+    } else if (kind == 1) {
+      h.a_planes = reinterpret_cast<const float*>(vp) + r0 * kpad;
+    } else {
+      h.a_planes = reinterpret_cast<const uint16_t*>(vp) + r0 * kpad;
+      h.a_planes_lo = reinterpret_cast<const uint16_t*>(vp) + nrv * kpad + r0 * kpad;
+    }
+  };
   G.drop_kept_planes();
   for (long a0 = 0; a0 < v; a0 += ab) {
     const long na = std::min<long>(ab, v - a0);
     const long nb = v - a0;
+    const long ntri = na * (na + 1) / 2;
     // V[a',b',c,d] = sum_Q L^[a0+a',c,Q] R^[a0+b',d,Q]   (m = c + v a', n = d + v b', k = Q)
-    GemmOp g;
-    g.M = (int)(na * v); g.N = (int)(nb * v); g.K = nq;
-    g.A = Lt + v * a0; g.a_s0 = 1; g.a_k = vv;
-    g.B = Rt + v * a0; g.b_s0 = 1; g.b_k = vv;
-    g.alpha = 1.0; g.beta = 0.0;
     PlaneOut po;
     po.hi = vp;
     po.lo = kind == 0 ? static_cast<void*>(reinterpret_cast<uint16_t*>(vp) + nrv * kpad) : nullptr;
-    po.ld = kpad; po.dm = (int)v; po.dn = (int)v; po.ra = 1; po.rb = na; po.kn = v;
-    G.run_planes(g, kind, &po);
-    // R2r[a0+a',a0+b',i,j] += w sum_{cd} V[a',b',c,d] T[c,d,i,j] on plane rows a' + na b'
-    for (int part = 0; part < 2; ++part) {
-      const long b0 = part == 0 ? 0 : na;           // diagonal block, then b' >= na
-      const long nbp = part == 0 ? na : nb - na;
-      if (nbp <= 0) continue;
-      const long r0 = na * b0;
-      GemmOp h;
-      h.M = (int)(na * nbp); h.N = (int)oo; h.K = (int)vv;
-      h.a_planes_ld = kpad;
-      if (kind == 2) {
-        h.a_planes = reinterpret_cast<const double*>(vp) + r0 * kpad;
-      } else if (kind == 1) {
-        h.a_planes = reinterpret_cast<const float*>(vp) + r0 * kpad;
-      } else {
-        h.a_planes = reinterpret_cast<const uint16_t*>(vp) + r0 * kpad;
-        h.a_planes_lo = reinterpret_cast<const uint16_t*>(vp) + nrv * kpad + r0 * kpad;
-      }
-      h.B = T2x; h.b_s0 = 1; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = oo;
-      h.C = R2r + a0 + (a0 + b0) * v; h.c_ms0 = 1; h.c_md0 = (int)na; h.c_ms1 = v;
-      h.c_ns0 = vv; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
-      h.alpha = part == 0 ? 0.5 : 1.0; h.beta = 1.0;
-      h.b_keep = true;
-      G.run_planes(h, kind);
+    po.ld = kpad; po.dm = (int)v; po.dn = (int)v; po.kn = v;
+    GemmOp g;
+    g.M = (int)(na * v); g.N = (int)(na * v); g.K = nq;
+    g.A = Lt + v * a0; g.a_s0 = 1; g.a_k = vv;
+    g.B = Rt + v * a0; g.b_s0 = 1; g.b_k = vv;
+    g.alpha = 1.0; g.beta = 0.0;
+    po.tri = 1;
+    G.run_planes(g, kind, &po);                                   // the triangle a' <= b' < na
+    if (nb > na) {
+      GemmOp go = g;
+      go.N = (int)((nb - na) * v);
+      go.B = Rt + v * (a0 + na);
+      PlaneOut pq = po;
+      pq.tri = 0;
+      pq.ra = 1; pq.rb = na;
+      pq.hi = kind == 0 ? static_cast<void*>(reinterpret_cast<uint16_t*>(vp) + ntri * kpad)
+                        : static_cast<void*>(vp + ntri * kpad * esz);
+      pq.lo = kind == 0 ? static_cast<void*>(reinterpret_cast<uint16_t*>(vp) + nrv * kpad + ntri * kpad) : nullptr;
+      G.run_planes(go, kind, &pq);                                // b' >= na
     }
+    // R2r[a0+a',a0+b',i,j] += w sum_{cd} V[a',b',c,d] T[c,d,i,j]
+    GemmOp h;
+    h.N = (int)oo; h.K = (int)vv;
+    h.B = T2x; h.b_s0 = 1; h.b_d0 = 1 << 30; h.b_s1 = 0; h.b_k = oo;
+    h.b_keep = true;
+    if (nb > na) {
+      GemmOp ho = h;
+      ho.M = (int)(na * (nb - na));
+      planes_at(ho, ntri);
+      ho.C = R2r + a0 + (a0 + na) * v; ho.c_ms0 = 1; ho.c_md0 = (int)na; ho.c_ms1 = v;
+      ho.c_ns0 = vv; ho.c_nd0 = 1 << 30; ho.c_ns1 = 0;
+      ho.alpha = 1.0; ho.beta = 1.0;
+      G.run_planes(ho, kind);
+    }
+    h.M = (int)ntri;
+    planes_at(h, 0);
+    h.C = ytri; h.c_ms0 = 1; h.c_md0 = 1 << 30; h.c_ms1 = 0; h.c_ns0 = ntri; h.c_nd0 = 1 << 30; h.c_ns1 = 0;
+    h.alpha = 1.0; h.beta = 0.0;
+    G.run_planes(h, kind);
+    const long n = ntri * oo;
+    hipLaunchKernelGGL(ladder_tri_scatter_kernel, dim3((unsigned)std::min<long>((n + 255) / 256, 65535)), dim3(256),
+                       0, ops.stream, (int)na, v, a0, ntri, oo, ytri, R2r);
+    ECG_HIP(hipGetLastError());
   }
   G.drop_kept_planes();
   ops.arena.release_to(mark);

@@ -114,12 +114,15 @@ struct PackedOp {
 // goes to plane row a' ra + b' rb, K position c + kn d, in the layout of the product
 // kind's packing (BF16 hi and lo planes, the FP32 or the FP64 plane; rows of ld elements).
 // The product stores with lanes along m, i.e. along consecutive positions of a row.
+// With tri, only elements with a' <= b' are stored, at row b' (b' + 1) / 2 + a' (ra, rb
+// unused), and tiles that hold none are not computed.
 struct PlaneOut {
   void* hi = nullptr;   // BF16 hi plane, or the FP32 / FP64 plane
   void* lo = nullptr;   // BF16 lo plane
   long ld = 0;
   int dm = 1, dn = 1;
   long ra = 0, rb = 0, kn = 0;
+  int tri = 0;
 };
 
 // Precision of the products: FP64 MFMA, FP32 MFMA (inputs rounded to FP32,

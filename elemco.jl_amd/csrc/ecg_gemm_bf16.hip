@@ -195,7 +195,7 @@ struct X3Params {
   int cplain; long ldc_ws;
   int box_class, box_v;   // boxed output (see PackedOp)
   // plane output (see PlaneOut): 0 none, 1 BF16 hi/lo planes, 2 FP32 plane, 3 FP64 plane
-  int pout_kind, pout_dm, pout_dn;
+  int pout_kind, pout_dm, pout_dn, pout_tri;
   long pout_ra, pout_rb, pout_kn, pout_ld;
   void* pout_hi;
   void* pout_lo;
@@ -207,7 +207,8 @@ struct X3Params {
 // slabs with position 8 (k % 4) + (k % 32) / 4 holding k; FP64 16-wide slabs with
 // 4 (k % 4) + (k % 16) / 4.
 __device__ __forceinline__ void plane_put(const X3Params& p, int a, int c, int b, int d, double x) {
-  const long row = (long)a * p.pout_ra + (long)b * p.pout_rb;
+  if (p.pout_tri && a > b) return;
+  const long row = p.pout_tri ? (long)b * (b + 1) / 2 + a : (long)a * p.pout_ra + (long)b * p.pout_rb;
   const long k = (long)c + p.pout_kn * (long)d;
   if (p.pout_kind == 3) {
     const long pos = (k & ~15L) | ((k & 3) << 2) | ((k >> 2) & 3);
@@ -230,6 +231,14 @@ __device__ __forceinline__ long box_off(int m, int n, int cls, int bv) {
   const long nt = (bv + 7) >> 3;
   return (((long)(x1 >> 3) + nt * ((x2 >> 3) + nt * (long)(x3 >> 3))) << 9) + (x1 & 7) + ((x2 & 7) << 3) +
          ((x3 & 7) << 6);
+}
+
+// A triangular plane output (PlaneOut::tri) stores nothing of a tile whose every row has a'
+// greater than every column's b': such a workgroup returns before its K loop.
+__device__ __forceinline__ bool x3_tile_skipped(const X3Params& p, int bm, int bn) {
+  if (!p.pout_tri) return false;
+  const int nlast = min(bn + XN, p.N) - 1;
+  return bm / p.pout_dm > nlast / p.pout_dn;
 }
 
 __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 3); }
@@ -329,6 +338,7 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
+  if (x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;   // batch index (packed products)
   const int kbeg = kz * p.kchunk;
@@ -550,6 +560,7 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
+  if (x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;
   const int kbeg = kz * p.kchunk;
@@ -695,6 +706,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
+  if (x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;
   const int kbeg = kz * p.kchunk;
@@ -805,6 +817,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
+  if (x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;
   const int kbeg = kz * p.kchunk;
@@ -943,9 +956,20 @@ void GemmEngine::run_planes(const GemmOp& op, int kind, const PlaneOut* po) {
     pa.src[s] = pa.src[0]; pb.src[s] = pb.src[0];
     pa.kp[s + 1] = pb.kp[s + 1] = kpad;
   }
-  flops += 2LL * op.M * op.N * ktot;
-  bytes += 8.0 * ((double)(op.M + op.N) * ktot + (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N);
   const int tiles_m = (op.M + XM - 1) / XM, tiles_n = (op.N + XN - 1) / XN;
+  long long mn = (long long)op.M * op.N;
+  if (po && po->tri) {
+    // the tiles a triangular plane output computes (x3_tile_skipped)
+    mn = 0;
+    for (int tm = 0; tm < tiles_m; ++tm)
+      for (int tn = 0; tn < tiles_n; ++tn) {
+        const int nlast = std::min((tn + 1) * XN, op.N) - 1;
+        if ((tm * XM) / po->dm > nlast / po->dn) continue;
+        mn += (long long)(std::min((tm + 1) * XM, op.M) - tm * XM) * (nlast + 1 - tn * XN);
+      }
+  }
+  flops += 2LL * mn * ktot;
+  bytes += 8.0 * ((double)(op.M + op.N) * ktot + (op.beta != 0.0 ? 2.0 : 1.0) * op.M * op.N);
   const long Mp = (long)tiles_m * XM, Np = (long)tiles_n * XN;
   // bytes per element of a packed operand (two BF16 planes or one FP32 plane: 4; FP64: 8)
   const size_t esz = kind == 2 ? 8 : 4;
@@ -1056,7 +1080,7 @@ void GemmEngine::run_planes(const GemmOp& op, int kind, const PlaneOut* po) {
     x.pout_kind = kind == 2 ? 3 : (kind == 1 ? 2 : 1);
     x.pout_dm = po->dm; x.pout_dn = po->dn;
     x.pout_ra = po->ra; x.pout_rb = po->rb; x.pout_kn = po->kn; x.pout_ld = po->ld;
-    x.pout_hi = po->hi; x.pout_lo = po->lo;
+    x.pout_hi = po->hi; x.pout_lo = po->lo; x.pout_tri = po->tri;
   }
   if (kind == 2) {
     if (to) hipLaunchKernelGGL((gemm_f64p_dma_kernel<true>), grid, dim3(512), 0, stream, x);

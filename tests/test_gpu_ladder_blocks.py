@@ -9,10 +9,11 @@ residual and kept for every block (GemmOp::b_keep).  The context here is made
 with a workspace small enough that the ladder runs in several blocks, each with
 a diagonal and an off-diagonal part, so the kept planes are reused.
 
-With v^2 a multiple of 32, o^2 >= 128 and v >= 64 (the o=12 v=64 and o=40 v=120 cases
-here) the V blocks are written straight into the contraction's packed planes (the
-factors transposed once per residual to [c, a, Q]); o=20 v=50 runs the V blocks in FP64
-and packs them.
+With o^2 >= 128 and v >= 64 (the o=12 and o=40 cases here) the V blocks are written
+straight into the contraction's packed planes (the factors transposed once per residual
+to [c, a, Q]), each block's diagonal part as its a' <= b' triangle contracted into a
+temporary and added with weights 1 and 1/2; v = 66 has v^2 not a multiple of 32 (zeroed K
+padding in the planes); o=20 v=50 runs the V blocks in FP64 and packs them.
 
 The workspace is the residual's other arena tensors at the time of the ladder
 (five o^2 v^2 tensors, W o^4, d_oovo o^3 v, Z v o Q and the Fock-like blocks)
@@ -45,6 +46,8 @@ def _workspace(o, v, q, rows):
     ((20, 50, 80, 31), 16, E.ECG_FP32, 1e-5),
     ((12, 64, 100, 33), 16, E.ECG_BF16X3, 1e-4),
     ((12, 64, 100, 33), 16, E.ECG_FP32, 1e-5),
+    ((12, 66, 100, 34), 16, E.ECG_BF16X3, 1e-4),
+    ((12, 66, 100, 34), 20, E.ECG_FP64, 0.0),
     ((40, 120, 100, 32), 68, E.ECG_FP64, 0.0),
 ])
 def test_multi_block_ladder_residual_matches_oracle(cfg, rows, prec, tol):
