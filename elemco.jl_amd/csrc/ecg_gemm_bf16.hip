@@ -249,10 +249,10 @@ __device__ __forceinline__ int swz(int row, int ch) { return ch ^ ((row >> 1) & 
 // issued after a store to C could alias it, which would serialise every element).
 // L64: the FP64 16x16x4 MFMA's layout (slot r of lane (li, lk) holds row lk + 4 r of
 // column li) instead of the FP32/BF16 one (row 4 lk + r).
-template <bool TO, typename A4 = f4_t, bool L64 = false>
+template <bool TO, typename A4 = f4_t, bool L64 = false, bool PO = false>
 __device__ __forceinline__ void x3_epilogue(const X3Params& p, const A4 (&acc)[4][4], int bm, int bn, int wm,
                                             int wn, int li, int lk, int kz, long zb) {
-  if (p.pout_kind) {
+  if constexpr (PO) {
     // plane output (the host launches it with TO, lanes along m): (a', c) once per row
     // block and (b', d) once per column, 32-bit divisions
 #pragma unroll
@@ -338,7 +338,6 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
-  if (x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;   // batch index (packed products)
   const int kbeg = kz * p.kchunk;
@@ -550,7 +549,7 @@ __device__ __forceinline__ void issue_bf16x3_slab(const X3Params& p, u32x4_t (&A
 // gemm_bf16x3_kernel.  The three buffers are separate LDS objects and the loop is
 // unrolled by three, so the compiler sees which buffer a DMA writes and which one
 // the MFMAs read (no conservative wait on the loads in flight).
-template <bool TO, bool MS>
+template <bool TO, bool MS, bool PO = false>
 __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
   __shared__ u32x4_t A0[2][XM][4], A1[2][XM][4], A2[2][XM][4];   // [plane][row][16-byte chunk]
   __shared__ u32x4_t B0[2][XN][4], B1[2][XN][4], B2[2][XN][4];
@@ -560,7 +559,7 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
-  if (x3_tile_skipped(p, bm, bn)) return;
+  if (PO && x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;
   const int kbeg = kz * p.kchunk;
@@ -644,7 +643,7 @@ __global__ __launch_bounds__(512, 1) void gemm_bf16x3_dma_kernel(X3Params p) {
     barrier();
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
-  x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
+  x3_epilogue<TO, f4_t, false, PO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
 }
 
 // FP32 planes and row strides (in floats) of the segment holding slab k0: the packed
@@ -696,7 +695,7 @@ __device__ __forceinline__ int plane_swz(int r) { return (r & 4) | ((r & 1) << 1
 // floats = 8 chunks of 16 bytes, chunk c of row r stored at slot c ^ plane_swz(r).  Per slab
 // and wave: 16 reads of 16 bytes and 128 v_mfma_f32_16x16x4f32 (eight k steps of the 4x4
 // tiles).
-template <bool TO, bool MS>
+template <bool TO, bool MS, bool PO = false>
 __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
   __shared__ u32x4_t A0[XM][8], A1[XM][8], A2[XM][8];   // [row][16-byte chunk]
   __shared__ u32x4_t B0[XN][8], B1[XN][8], B2[XN][8];
@@ -706,7 +705,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
-  if (x3_tile_skipped(p, bm, bn)) return;
+  if (PO && x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;
   const int kbeg = kz * p.kchunk;
@@ -797,7 +796,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f32p_dma_kernel(X3Params p) {
     barrier();
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
-  x3_epilogue<TO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
+  x3_epilogue<TO, f4_t, false, PO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
 }
 
 constexpr int XK64 = 16;   // K slab of the FP64 packed product (128 bytes per row, as FP32's 32)
@@ -807,7 +806,7 @@ constexpr int XK64 = 16;   // K slab of the FP64 packed product (128 bytes per r
 // three LDS buffers, SGPR-base direct-to-LDS slab loads and swizzle of gemm_f32p_dma_kernel
 // with 16-wide K slabs of doubles and v_mfma_f64_16x16x4f64 (64 per slab and wave, as
 // many MFMA cycles as the FP32 kernel's 128).
-template <bool TO, bool MS = false>
+template <bool TO, bool MS = false, bool PO = false>
 __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
   __shared__ u32x4_t As[3][XM][8];   // [buffer][row][16-byte chunk]
   __shared__ u32x4_t Bs[3][XN][8];
@@ -817,7 +816,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
   int tm, tn;
   tile_coords(blockIdx.x, gridDim.x, p.tiles_m, tm, tn);
   const int bm = tm * XM, bn = tn * XN;
-  if (x3_tile_skipped(p, bm, bn)) return;
+  if (PO && x3_tile_skipped(p, bm, bn)) return;
   const int kz = blockIdx.z;
   const long zb = blockIdx.y;
   const int kbeg = kz * p.kchunk;
@@ -918,7 +917,7 @@ __global__ __launch_bounds__(512, 1) void gemm_f64p_dma_kernel(X3Params p) {
     buf = buf == 2 ? 0 : buf + 1;
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // no load into LDS outlives the loop
-  x3_epilogue<TO, d4_t, true>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
+  x3_epilogue<TO, d4_t, true, PO>(p, acc, bm, bn, wm, wn, li, lk, kz, zb);
 }
 
 }  // namespace
@@ -1081,8 +1080,11 @@ void GemmEngine::run_planes(const GemmOp& op, int kind, const PlaneOut* po) {
     x.pout_dm = po->dm; x.pout_dn = po->dn;
     x.pout_ra = po->ra; x.pout_rb = po->rb; x.pout_kn = po->kn; x.pout_ld = po->ld;
     x.pout_hi = po->hi; x.pout_lo = po->lo; x.pout_tri = po->tri;
-  }
-  if (kind == 2) {
+    // plane output: lanes along m, the direct-to-LDS kernels
+    if (kind == 2) hipLaunchKernelGGL((gemm_f64p_dma_kernel<true, false, true>), grid, dim3(512), 0, stream, x);
+    else if (kind == 1) hipLaunchKernelGGL((gemm_f32p_dma_kernel<true, false, true>), grid, dim3(512), 0, stream, x);
+    else hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<true, false, true>), grid, dim3(512), 0, stream, x);
+  } else if (kind == 2) {
     if (to) hipLaunchKernelGGL((gemm_f64p_dma_kernel<true>), grid, dim3(512), 0, stream, x);
     else hipLaunchKernelGGL((gemm_f64p_dma_kernel<false>), grid, dim3(512), 0, stream, x);
   } else if (kind == 1) {
