@@ -326,9 +326,12 @@ void Context::ladder_df(Prec pr, const double* Lf, const double* Rf, const doubl
   df_ladder_half(ops, nv, no, nq, pr, Lf, Rf, T2x, R2r);
 }
 
-#ifndef LADDER_MAX_ROWS
-#define LADDER_MAX_ROWS 64
-#endif
+// a-rows per block of the fused ladder (fewer when the arena is short).  With the diagonal
+// blocks formed as triangles every pair a <= b is computed once whatever the block, so the
+// size only trades launch count against product size: the c2 iteration (o=40 v=360) takes
+// FP64 1708 / 1702 / 1701 ms and BF16 hi/lo 398 / 401 / 406 ms with at most 32 / 64 / 128
+// rows (the arena holds about 55 rows of FP64 planes there)
+constexpr long kLadderMaxRows = 64;
 
 // R2r[a0+a',a0+b',ij] += w Y[t,ij] for the triangle rows t = b'(b'+1)/2 + a' (a' <= b' < na),
 // w = 1 off the diagonal and 1/2 on it (the later symmetrisation adds the transposed half)
@@ -393,7 +396,7 @@ static bool df_ladder_half_planes(TensorOps& ops, long nv, long no, int nq, Prec
   // off-diagonal rows, plus 256 rows that a contraction's last tile reads (results
   // discarded); the temporary of the triangle's contraction
   const long free_b = (long)ops.arena.free_bytes() - 256 * kpad * esz;
-  long ab = std::min<long>(v, LADDER_MAX_ROWS);
+  long ab = std::min<long>(v, kLadderMaxRows);
   auto need = [&](long r) { return (r * v + r * (r + 1) / 2) * kpad * esz + 8 * (r * (r + 1) / 2) * oo; };
   while (ab > 1 && need(ab) > (long)(0.8 * (double)free_b)) --ab;
   const long nrv = ab * v + ab * (ab + 1) / 2 + 256;
