@@ -66,13 +66,14 @@ constexpr int BX1 = TT + 1, BX2 = 76;
 // and c) are 64-term dot products of X with a slice, four lanes per entry and
 // a two-step shuffle.  The energy sum K.X is accumulated per thread over the
 // whole triple and reduced once; it is recorded with the first output block.
-// All sums run in a fixed order (deterministic).  X is formed in the type of K
-// (FP32 for the BF16 hi/lo path); the sums with W and Y are FP64.
+// All sums run in a fixed order (deterministic).  X and the staged W / Y slices are held
+// in the type of K (FP32 for the FP32 and BF16 hi/lo paths: 26 instead of 36 KB of LDS, five
+// workgroups per CU instead of four); the sums with W and Y are FP64.
 template <typename KT>
 __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
   __shared__ KT box[6][TT * BX2];
   __shared__ KT xs[TT * BX2];                // X of the current output block
-  __shared__ double wsl[2][3][6][TT * TT];   // [W|Y][occupied pair][ordered tile pair][x + 8 y]
+  __shared__ KT wsl[2][3][6][TT * TT];       // [W|Y][occupied pair][ordered tile pair][x + 8 y]
   __shared__ double esl[3][TT];              // virtual orbital energies of tiles T[0], T[1], T[2]
   __shared__ double ered[4];
   const int nt = p.ntile;
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
     const int c1 = T[pa] * TT + x, c2 = T[pb] * TT + y;
     const double* src = wy == 0 ? p.W : p.Y;
     const bool ok = c1 < v && c2 < v;
-    (&wsl[0][0][0][0])[u] = ok ? src[ok ? c1 + v * c2 + ooff[role] : 0] : 0.0;
+    (&wsl[0][0][0][0])[u] = ok ? (KT)src[ok ? c1 + v * c2 + ooff[role] : 0] : KT(0);
   }
   if (t < 3 * TT) {
     const int x = T[t / TT] * TT + t % TT;
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
       // entry didx of vector dvec: role 0 sums over (b,c), role 1 over (a,c), role 2 over (a,b)
       const int sl = drole == 0 ? pair(perm[o][1], perm[o][2])
                                 : (drole == 1 ? pair(perm[o][0], perm[o][2]) : pair(perm[o][0], perm[o][1]));
-      const double* ws = wsl[dwy][drole][sl];
+      const KT* ws = wsl[dwy][drole][sl];
       double sm = 0.0;
 #pragma unroll
       for (int u2 = 0; u2 < 2; ++u2)
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
           if (drole == 0) { xi = didx + BX1 * u1 + BX2 * u; wi = u1 + TT * u; }
           else if (drole == 1) { xi = u1 + BX1 * didx + BX2 * u; wi = u1 + TT * u; }
           else { xi = u1 + BX1 * u + BX2 * didx; wi = u1 + TT * u; }
-          sm += (double)xs[xi] * ws[wi];
+          sm += (double)xs[xi] * (double)ws[wi];
         }
       sm += __shfl_xor(sm, 1, 64);
       sm += __shfl_xor(sm, 2, 64);
