@@ -67,10 +67,11 @@ constexpr int BX1 = TT + 1, BX2 = 76;
 // a two-step shuffle.  The energy sum K.X is accumulated per thread over the
 // whole triple and reduced once; it is recorded with the first output block.
 // All sums run in a fixed order (deterministic).  X and the staged W / Y slices are held
-// in the type of K (FP32 for the FP32 and BF16 hi/lo paths: 26 instead of 36 KB of LDS, five
-// workgroups per CU instead of four); the sums with W and Y are FP64.
+// in the type of K (FP32 for the FP32 and BF16 hi/lo paths: 26 instead of 36 KB of LDS, and
+// with at most 80 VGPRs six workgroups per CU instead of four: 0.276 against 0.354 ms per
+// triplet at o=40 v=360); the sums with W and Y are FP64.
 template <typename KT>
-__global__ __launch_bounds__(256) void triples_combine_kernel(CombArgs<KT> p) {
+__global__ __launch_bounds__(256, 6) void triples_combine_kernel(CombArgs<KT> p) {
   __shared__ KT box[6][TT * BX2];
   __shared__ KT xs[TT * BX2];                // X of the current output block
   __shared__ KT wsl[2][3][6][TT * TT];       // [W|Y][occupied pair][ordered tile pair][x + 8 y]
