@@ -399,6 +399,10 @@ static bool df_ladder_half_planes(TensorOps& ops, long nv, long no, int nq, Prec
   long ab = std::min<long>(v, kLadderMaxRows);
   auto need = [&](long r) { return (r * v + r * (r + 1) / 2) * kpad * esz + 8 * (r * (r + 1) / 2) * oo; };
   while (ab > 1 && need(ab) > (long)(0.8 * (double)free_b)) --ab;
+  if (need(ab) > (long)(0.8 * (double)free_b)) {   // not even one a-row of planes: FP64 V blocks
+    ops.arena.release_to(mark);
+    return false;
+  }
   const long nrv = ab * v + ab * (ab + 1) / 2 + 256;
   char* vp = reinterpret_cast<char*>(ops.arena.take((nrv * kpad * esz + 7) / 8));
   double* ytri = ops.arena.take(ab * (ab + 1) / 2 * oo);
