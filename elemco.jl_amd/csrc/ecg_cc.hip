@@ -57,38 +57,47 @@ __global__ void update_singles_kernel(int nv, int no, const double* __restrict__
 // Per-(i,j) block sums  d = sum_ab X[a,b,i,j] Y[a,b,i,j],  e = sum_ab X[b,a,i,j] Y[a,b,i,j].
 // The v x v block is walked in 32 x 32 tiles: X and Y of tile (A, B) and X of the mirrored
 // tile (B, A) are read along their unit-stride index (coalesced), the mirrored tile is
-// transposed through LDS.  Each thread keeps 4 elements of a tile (256 threads).
-__global__ __launch_bounds__(256) void cs_dots_kernel(int nv, const double* __restrict__ X,
-                                                      const double* __restrict__ Y, double* part) {
-  __shared__ double xt[32][33];   // mirrored tile, [b][a] as read
-  __shared__ double sd[4], se[4];
+// transposed through LDS.  Each thread keeps 4 elements of a tile; four groups of 256
+// threads walk alternate tiles (one workgroup per pair: at o = 16 a single 256-thread group
+// per pair left one workgroup per CU waiting on its loads).
+__global__ __launch_bounds__(1024) void cs_dots_kernel(int nv, const double* __restrict__ X,
+                                                       const double* __restrict__ Y, double* part) {
+  __shared__ double xt[4][32][33];   // per group: mirrored tile, [b][a] as read
+  __shared__ double sd[16], se[16];
   const long off = (long)blockIdx.x * nv * nv;
   const double* x = X + off;
   const double* y = Y + off;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
-  const int nt = (nv + 31) / 32;
+  // four groups of 256 threads take tiles t0 + g of the (tb, ta) walk; every thread runs the
+  // same number of steps (a group past the last tile idles), so all reach every barrier
+  const int g = threadIdx.x >> 8, tl = threadIdx.x & 255;
+  const int tx = tl & 31, ty = tl >> 5;   // 32 x 8 threads per group
+  const int nt = (nv + 31) / 32, ntt = nt * nt;
   double d = 0.0, e = 0.0;
-  for (int tb = 0; tb < nt; ++tb)
-    for (int ta = 0; ta < nt; ++ta) {
-      // mirrored tile: rows b in tile tb (unit stride), columns a in tile ta
-      __syncthreads();
+  for (int t0 = 0; t0 < ntt; t0 += 4) {
+    const int t = t0 + g;
+    const bool on = t < ntt;
+    const int tb = on ? t / nt : 0, ta = on ? t - tb * nt : 0;
+    // mirrored tile: rows b in tile tb (unit stride), columns a in tile ta
+    __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int b = tb * 32 + tx, a = ta * 32 + ty + 8 * k;
-        xt[ty + 8 * k][tx] = (a < nv && b < nv) ? x[b + (long)a * nv] : 0.0;
-      }
-      __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+      const int b = tb * 32 + tx, a = ta * 32 + ty + 8 * k;
+      xt[g][ty + 8 * k][tx] = (on && a < nv && b < nv) ? x[b + (long)a * nv] : 0.0;
+    }
+    __syncthreads();
+    if (on) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int a = ta * 32 + tx, b = tb * 32 + ty + 8 * k;
         if (a < nv && b < nv) {
-          const long t = a + (long)b * nv;
-          const double yv = y[t];
-          d += x[t] * yv;
-          e += xt[tx][ty + 8 * k] * yv;   // X[b, a]
+          const long ti = a + (long)b * nv;
+          const double yv = y[ti];
+          d += x[ti] * yv;
+          e += xt[g][tx][ty + 8 * k] * yv;   // X[b, a]
         }
       }
     }
+  }
   for (int s = 32; s > 0; s >>= 1) {
     d += __shfl_down(d, s, 64);
     e += __shfl_down(e, s, 64);
@@ -96,8 +105,10 @@ __global__ __launch_bounds__(256) void cs_dots_kernel(int nv, const double* __re
   if ((threadIdx.x & 63) == 0) { sd[threadIdx.x >> 6] = d; se[threadIdx.x >> 6] = e; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sd[0] + sd[1] + sd[2] + sd[3];
-    part[2 * blockIdx.x + 1] = se[0] + se[1] + se[2] + se[3];
+    double ds = 0.0, es = 0.0;
+    for (int w = 0; w < 16; ++w) { ds += sd[w]; es += se[w]; }
+    part[2 * blockIdx.x] = ds;
+    part[2 * blockIdx.x + 1] = es;
   }
 }
 
@@ -850,7 +861,7 @@ void Context::mp2_amplitudes(double* T1d, double* T2d_) {
 
 // Per-(i,j) partial sums [d, e] of two v x v x o x o tensors (pair_dots in ecg_cc.h)
 void launch_pair_dots(hipStream_t s, int nv, long npairs, const double* X, const double* Y, double* part) {
-  hipLaunchKernelGGL(cs_dots_kernel, dim3((unsigned)npairs), dim3(256), 0, s, nv, X, Y, part);
+  hipLaunchKernelGGL(cs_dots_kernel, dim3((unsigned)npairs), dim3(1024), 0, s, nv, X, Y, part);
   ECG_HIP(hipGetLastError());
 }
 
