@@ -1031,18 +1031,24 @@ void Context::state_step(int method, Prec pr, const SolverOpts& o, double* info,
     if (o.graphs && !ops.gemm.timed) residual_graph(method, p, T1.d(), T2.d(), r1, r2);
     else residual(method, p, T1.d(), T2.d(), r1, r2);
   };
-  resid(pr, R1.d(), R2.d());
-  if (pr != Prec::F64) {
-    if (refresh) {
-      // delta = R_64 - R_low at this T; continue with the exact residual R_low + delta = R_64
+  if (pr != Prec::F64 && refresh) {
+    // the exact residual R_64 at this T drives this iteration; delta = R_64 - R_low corrects
+    // the following low-precision ones.  When R_64 already meets thr (the iteration that
+    // converges) R_low is not formed: a further iteration would refresh again anyway
+    // (the refresh follows every corrected norm below 100 thr)
+    resid(Prec::F64, R1.d(), R2.d());
+    const double nr = 2.0 * doubles_dot(R2.d(), R2.d(), 0.5) + 2.0 * ops.dot_host(n1, R1.d(), R1.d());
+    if (nr >= o.thr) {
       D1.ensure(sizeof(double) * n1);
       D2.ensure(sizeof(double) * nvvoo());
-      resid(Prec::F64, D1.d(), D2.d());
-      ops.axpby(n1, -1.0, R1.d(), 1.0, D1.d());
-      ops.axpby(nvvoo(), -1.0, R2.d(), 1.0, D2.d());
+      resid(pr, D1.d(), D2.d());
+      ops.axpby(n1, 1.0, R1.d(), -1.0, D1.d());
+      ops.axpby(nvvoo(), 1.0, R2.d(), -1.0, D2.d());
       have_delta = true;
     }
-    if (have_delta) {
+  } else {
+    resid(pr, R1.d(), R2.d());
+    if (pr != Prec::F64 && have_delta) {
       ops.axpby(n1, 1.0, D1.d(), 1.0, R1.d());
       ops.axpby(nvvoo(), 1.0, D2.d(), 1.0, R2.d());
     }

@@ -406,17 +406,22 @@ void lambda_solve(Context& c, int method, Prec pr, const SolverOpts& o, const do
     const bool refresh =
         mixed && !fp64_only && (refresh_next || (o.refine_every > 0 && (it % o.refine_every) == 0));
     const bool exact = !mixed || refresh || fp64_only;
-    residual(c, F, method, fp64_only ? Prec::F64 : pr, U1.d(), U2.d(), R1.d(), R2.d());
-    if (mixed && !fp64_only) {
-      if (refresh) {
+    if (mixed && !fp64_only && refresh) {
+      // exact residual first; R_low (for delta = R_64 - R_low) only when R_64 does not yet
+      // meet thr, as in Context::state_step
+      residual(c, F, method, Prec::F64, U1.d(), U2.d(), R1.d(), R2.d());
+      const double nr = (2.0 / 3.0) * c.doubles_dot(R2.d(), R2.d(), -0.5) + 0.5 * ops.dot_host(n1, R1.d(), R1.d());
+      if (nr >= o.thr) {
         D1.ensure(sizeof(double) * n1);
         D2.ensure(sizeof(double) * n2);
-        residual(c, F, method, Prec::F64, U1.d(), U2.d(), D1.d(), D2.d());
-        ops.axpby(n1, -1.0, R1.d(), 1.0, D1.d());
-        ops.axpby(n2, -1.0, R2.d(), 1.0, D2.d());
+        residual(c, F, method, pr, U1.d(), U2.d(), D1.d(), D2.d());
+        ops.axpby(n1, 1.0, R1.d(), -1.0, D1.d());
+        ops.axpby(n2, 1.0, R2.d(), -1.0, D2.d());
         have_delta = true;
       }
-      if (have_delta) {
+    } else {
+      residual(c, F, method, fp64_only ? Prec::F64 : pr, U1.d(), U2.d(), R1.d(), R2.d());
+      if (mixed && !fp64_only && have_delta) {
         ops.axpby(n1, 1.0, D1.d(), 1.0, R1.d());
         ops.axpby(n2, 1.0, D2.d(), 1.0, R2.d());
       }
