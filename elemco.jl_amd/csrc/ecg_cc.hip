@@ -322,8 +322,9 @@ void Context::dress(const double* T1d) {
 // final symmetrisation R2 += R2r + R2r[b,a,j,i] restores the full term, because
 // P[a,b,i,j] = P[b,a,j,i] ((ac|bd) = (bd|ac)).  For a block of a the integrals
 // V[a',b',c,d] (b = a0 + b' >= a0) are assembled on MFMA and contracted at once;
-// the v^4 tensor never exists and only b >= a0 is formed (half the work).  Pairs
-// inside the diagonal block are formed twice and weighted by 1/2.
+// the v^4 tensor never exists and only b >= a0 is formed (half the work).  Inside the
+// diagonal block the fused path (df_ladder_half_planes) forms the pairs a <= b once
+// (weight 1/2 for a = b); the FP64-V path forms them twice, each weighted by 1/2.
 void Context::ladder(Prec pr, const double* T2x, double* R2r) {
   const Blocks& Rh = hermitian ? Ld : Rd;
   if (!capturing) ECG_HIP(hipEventRecord(ev[2], stream));

@@ -76,3 +76,45 @@ def test_multi_block_ladder_residual_matches_oracle(cfg, rows, prec, tol):
                 assert err < tol * scale
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(E.ECG_FP64, 0.0), (E.ECG_BF16X3, 1e-4)])
+def test_two_sided_factors_fused_ladder_matches_oracle(prec, tol):
+    """Non-Hermitian integrals (pq|rs) = sum_Q BL[p,q,Q] BR[r,s,Q] with BR != BL, at o=12 v=66
+    naux=100: the fused ladder then transposes both factors (the Hermitian residual passes
+    one factor twice).  The integrals keep the particle symmetry (pq|rs) = (rs|pq) of the
+    reference's similarity-transformed dumps (which the half ladder and the final
+    symmetrisation use) but not (pq|rs) = (qp|rs): B[p,q,P] is perturbed asymmetrically,
+    BR = B and BL = B D with D symmetric.  The oracle takes the same integrals as the
+    four-index array <pq|rs> = sum_Q BL[p,r,Q] BR[q,s,Q]; MP2 amplitudes of the oracle.
+    Bars as above."""
+    o, v, q, seed = 12, 66, 100, 35
+    sc = 0.7 * conditioned_scale(o, v, q)
+    _, h, B0, e0 = make_system(o, v, q, seed, sc)
+    rng = np.random.default_rng(seed)
+    Bns = B0 + 0.1 * float(np.std(B0)) * rng.standard_normal(B0.shape)
+    X = 0.05 * rng.standard_normal((q, q))
+    D = np.eye(q) + 0.5 * (X + X.T)
+    BR = np.asfortranarray(Bns)
+    BL = np.asfortranarray(np.einsum("pqP,PQ->pqQ", Bns, D))
+    ctx = E.Context(o, v, q, h, BL, BR, e0, workspace_bytes=_workspace(o, v, q, 16))
+    int2 = np.einsum("prQ,qsQ->pqrs", BL, BR, optimize=True)
+    assert np.allclose(int2, int2.transpose(1, 0, 3, 2))     # particle symmetry
+    ECo = O.ECInfo(O.FourIndexInts(int2), h, e0, o)
+    O.calc_fock_matrix(ECo)
+    O.calc_MP2(ECo)
+    T1, T2 = ECo.files["T_vo"], ECo.files["T_vvoo"]
+    try:
+        for dc in (False, True):
+            R1o, R2o = O.calc_cc_resid(ECo, T1, T2, dc=dc)
+            R1, R2 = ctx.cc_residual(T1, T2, E.ECG_DCSD if dc else E.ECG_CCSD, prec)
+            err = max(float(np.max(np.abs(R1 - R1o))), float(np.max(np.abs(R2 - R2o))))
+            scale = float(np.max(np.abs(R2o)))
+            print(f"  two-sided o={o} v={v} prec={prec} dc={dc}: max|R - R_oracle| = {err:.3e} "
+                  f"(max|R2| {scale:.3e})", flush=True)
+            if prec == E.ECG_FP64:
+                assert err < 1e-10
+            else:
+                assert err < tol * scale
+    finally:
+        ctx.close()
