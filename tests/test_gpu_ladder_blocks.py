@@ -47,6 +47,7 @@ def _workspace(o, v, q, rows):
     ((12, 64, 100, 33), 16, E.ECG_BF16X3, 1e-4),
     ((12, 64, 100, 33), 16, E.ECG_FP32, 1e-5),
     ((12, 66, 100, 34), 16, E.ECG_BF16X3, 1e-4),
+    ((12, 66, 100, 34), 16, E.ECG_FP32, 1e-5),
     ((12, 66, 100, 34), 20, E.ECG_FP64, 0.0),
     ((40, 120, 100, 32), 68, E.ECG_FP64, 0.0),
 ])
